@@ -1,0 +1,111 @@
+/*
+ * tsort.h -- C ABI of the MI355X-native (gfx950) radix sort with turbosort's API.
+ *
+ * Every pointer named d_* is device memory on the context's GPU.  `stream` is a
+ * hipStream_t passed as an opaque pointer (NULL = the default stream); all work
+ * of a call is enqueued on it and the call returns without waiting, except
+ * where noted.  No C++ or torch types cross this boundary.
+ *
+ * The reference crate (turbosort 0.2.2) is a Rust library with no FFI; each
+ * entry point below names the reference function it replaces.  Rust panics in
+ * the reference become status codes here, and ts_ctx_last_error() returns the
+ * reference's panic message for them, word for word.
+ */
+#ifndef TSORT_H
+#define TSORT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ts_status {
+  TS_OK = 0,
+  TS_ERR_BUFFER_TOO_SMALL = 1, /* dispatch.rs:78-83, radix/mod.rs:92-97 (panic) */
+  TS_ERR_LENGTH_MISMATCH = 2,  /* by_key.rs:137-141 (panic) */
+  TS_ERR_TOO_LONG = 3,         /* by_key.rs:190-194: order() stays empty above u32::MAX */
+  TS_ERR_INVALID_ARG = 4,
+  TS_ERR_HIP = 5,              /* a HIP runtime call failed; see ts_ctx_last_error */
+  TS_ERR_VERIFY_FAILED = 6,    /* an output failed sortedness or the multiset checksum */
+  TS_ERR_NO_DEVICE = 7
+} ts_status;
+
+/* The ten sealed SortableKey types (key.rs:64-77), in this order. */
+typedef enum ts_dtype {
+  TS_U8 = 0, TS_U16 = 1, TS_U32 = 2, TS_U64 = 3,
+  TS_I8 = 4, TS_I16 = 5, TS_I32 = 6, TS_I64 = 7,
+  TS_F32 = 8, TS_F64 = 9
+} ts_dtype;
+
+/* A device workspace: look-back state, digit histograms and grow-only scratch.
+ * One context per stream at a time; like &mut SortByKeyBuffer it is not
+ * thread-safe.  Separate contexts may run concurrently on separate streams. */
+typedef struct ts_ctx ts_ctx;
+
+ts_status ts_ctx_create(ts_ctx** out, int device);
+void ts_ctx_destroy(ts_ctx* ctx);
+/* Grow the workspace so that sorts of n keys of `dtype` (and by-key sorts of n
+ * records of `record_bytes`) allocate nothing.  Mirrors
+ * SortByKeyBuffer::with_capacity (by_key.rs:74-81). */
+ts_status ts_ctx_reserve(ts_ctx* ctx, ts_dtype dtype, size_t n, size_t record_bytes);
+/* Bytes of device memory the context holds. */
+size_t ts_ctx_workspace_bytes(const ts_ctx* ctx);
+/* Message for the last failed call on this context (reference panic text). */
+const char* ts_ctx_last_error(const ts_ctx* ctx);
+const char* ts_status_string(ts_status s);
+
+/* turbosort::sort (lib.rs:99-101): sort n keys of `dtype` in place, ascending
+ * in the type's total order (f32/f64: IEEE-754 totalOrder, key.rs:162-198). */
+ts_status ts_sort(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, void* stream);
+
+/* turbosort::sort_with_buffer (lib.rs:122-124): as ts_sort with the caller's
+ * scratch d_buf of buf_len elements.  buf_len < n is TS_ERR_BUFFER_TOO_SMALL
+ * with "buffer too small: need {n}, got {buf_len}". */
+ts_status ts_sort_with_buffer(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, void* d_buf,
+                              size_t buf_len, void* stream);
+
+/* turbosort::sort_by_key_with_buffer (lib.rs:196-202, by_key.rs:175-235) for
+ * POD records: record i starts at d_records + i*stride and its key, of type
+ * key_type, at byte key_offset.  Writes d_order_out[i] = original position of
+ * the record that sorts to i (SortByKeyBuffer::order, by_key.rs:108-110), the
+ * stable order.  If gather_records != 0 the records are then moved into that
+ * order in place (move_into_order, by_key.rs:315-334).  n > u32::MAX is
+ * TS_ERR_TOO_LONG.  stride == key width and key_offset == 0 sorts a plain key
+ * array by itself and yields its stable argsort. */
+ts_status ts_sort_by_key(ts_ctx* ctx, ts_dtype key_type, void* d_records, size_t n, size_t stride,
+                         size_t key_offset, uint32_t* d_order_out, int gather_records, void* stream);
+
+/* SortByKeyBuffer::apply_order (by_key.rs:136-144): d_other[i] <- d_other[order[i]]
+ * for n elements of elem_bytes bytes, in place.  order_len != n is
+ * TS_ERR_LENGTH_MISMATCH with the reference's message. */
+ts_status ts_apply_order(ts_ctx* ctx, const uint32_t* d_order, size_t order_len, void* d_other, size_t n,
+                         size_t elem_bytes, void* stream);
+
+/* ---- Utilities for tests and benchmarks (no reference counterpart) ---- */
+
+/* Fill d_out with n synthetic keys from SplitMix64 (Vigna), element i taking
+ * output i of the stream with state `seed`:
+ *   kind 0: u32 = high 32 bits;  kind 1: u64 = the output;
+ *   kind 2: u64 = zero-extended high 32 bits. */
+ts_status ts_fill_splitmix64(ts_ctx* ctx, void* d_out, size_t n, uint64_t seed, int kind, void* stream);
+
+/* Order-independent multiset checksum of n keys of `dtype` (sum and xor of a
+ * 64-bit mix of each key's bits) and the count of adjacent pairs out of order
+ * in the type's total order.  Synchronous: results land in host memory. */
+ts_status ts_checksum(ts_ctx* ctx, ts_dtype dtype, const void* d_data, size_t n, uint64_t out_sum_xor[2],
+                      uint64_t* out_unsorted_pairs, void* stream);
+
+/* Per-kernel timing of the last sort on this context (enabled with
+ * ts_ctx_set_profiling).  Fills up to `cap` (name, milliseconds, bytes moved
+ * algorithmically) triples; returns how many kernels were recorded.  The
+ * timings come from HIP events on the sort's own stream. */
+ts_status ts_ctx_set_profiling(ts_ctx* ctx, int enable);
+int ts_ctx_kernel_times(const ts_ctx* ctx, const char** names, float* ms, double* bytes, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSORT_H */

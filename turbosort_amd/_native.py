@@ -1,0 +1,74 @@
+"""ctypes binding of libtsort.so (the C ABI declared in include/tsort.h).
+
+The library is built in-tree (``python -m turbosort_amd.build``).  Loading
+fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtsort.so")
+
+# Every symbol include/tsort.h declares.
+EXPORTS = [
+    "ts_ctx_create", "ts_ctx_destroy", "ts_ctx_reserve", "ts_ctx_workspace_bytes", "ts_ctx_last_error",
+    "ts_status_string", "ts_sort", "ts_sort_with_buffer", "ts_sort_by_key", "ts_apply_order",
+    "ts_fill_splitmix64", "ts_checksum", "ts_ctx_set_profiling", "ts_ctx_kernel_times",
+]
+
+TS_OK = 0
+TS_ERR_BUFFER_TOO_SMALL = 1
+TS_ERR_LENGTH_MISMATCH = 2
+TS_ERR_TOO_LONG = 3
+TS_ERR_INVALID_ARG = 4
+TS_ERR_HIP = 5
+TS_ERR_VERIFY_FAILED = 6
+TS_ERR_NO_DEVICE = 7
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libtsort.so and declare its signatures (no GPU call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m turbosort_amd.build` "
+            "(turbosort_amd has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i32, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64
+    L.ts_ctx_create.argtypes = [ctypes.POINTER(vp), i32]
+    L.ts_ctx_create.restype = i32
+    L.ts_ctx_destroy.argtypes = [vp]
+    L.ts_ctx_destroy.restype = None
+    L.ts_ctx_reserve.argtypes = [vp, i32, sz, sz]
+    L.ts_ctx_reserve.restype = i32
+    L.ts_ctx_workspace_bytes.argtypes = [vp]
+    L.ts_ctx_workspace_bytes.restype = sz
+    L.ts_ctx_last_error.argtypes = [vp]
+    L.ts_ctx_last_error.restype = ctypes.c_char_p
+    L.ts_status_string.argtypes = [i32]
+    L.ts_status_string.restype = ctypes.c_char_p
+    L.ts_sort.argtypes = [vp, i32, vp, sz, vp]
+    L.ts_sort.restype = i32
+    L.ts_sort_with_buffer.argtypes = [vp, i32, vp, sz, vp, sz, vp]
+    L.ts_sort_with_buffer.restype = i32
+    L.ts_sort_by_key.argtypes = [vp, i32, vp, sz, sz, sz, vp, i32, vp]
+    L.ts_sort_by_key.restype = i32
+    L.ts_apply_order.argtypes = [vp, vp, sz, vp, sz, sz, vp]
+    L.ts_apply_order.restype = i32
+    L.ts_fill_splitmix64.argtypes = [vp, vp, sz, u64, i32, vp]
+    L.ts_fill_splitmix64.restype = i32
+    L.ts_checksum.argtypes = [vp, i32, vp, sz, ctypes.POINTER(u64), ctypes.POINTER(u64), vp]
+    L.ts_checksum.restype = i32
+    L.ts_ctx_set_profiling.argtypes = [vp, i32]
+    L.ts_ctx_set_profiling.restype = i32
+    L.ts_ctx_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float),
+                                      ctypes.POINTER(ctypes.c_double), i32]
+    L.ts_ctx_kernel_times.restype = i32
+    _lib = L
+    return L

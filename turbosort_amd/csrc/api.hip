@@ -1,0 +1,333 @@
+// C ABI (include/tsort.h): argument checks with the reference's panic
+// messages, workspace management and tier dispatch.
+//
+// Tiers (dispatch.rs:31-67): n <= 1 is a no-op; up to kSmallMax keys one
+// workgroup sorts in LDS (the reference's tiny/small network tiers); above
+// that the device LSD radix sort runs.  The reference's presorted scan and u8
+// counting sort are output-invisible shortcuts (presorted.rs:10-14): the
+// radix sort produces the same unique output for those inputs.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/tsort.h"
+#include "radix.h"
+
+struct ts_ctx {
+  int device = 0;
+  int num_cus = 256;
+  char* ws = nullptr;
+  size_t ws_bytes = 0;
+  std::string err;
+  ts::KernelClock clk;
+};
+
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// Runs a call with the context's device current and restores the caller's.
+struct DeviceScope {
+  int prev = -1;
+  explicit DeviceScope(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) hipSetDevice(dev);
+  }
+  ~DeviceScope() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) hipSetDevice(prev);
+  }
+};
+
+ts_status fail(ts_ctx* c, ts_status st, const std::string& msg) {
+  if (c) c->err = msg;
+  return st;
+}
+
+ts_status hip_fail(ts_ctx* c, hipError_t e, const char* where) {
+  return fail(c, TS_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+ts_status ensure(ts_ctx* c, size_t bytes) {
+  if (bytes <= c->ws_bytes) return TS_OK;
+  if (c->ws) {
+    hipError_t e = hipFree(c->ws);  // waits for work still using the old workspace
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+    if (e != hipSuccess) return hip_fail(c, e, "hipFree");
+  }
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&c->ws), bytes);
+  if (e != hipSuccess) {
+    c->ws = nullptr;
+    return hip_fail(c, e, "hipMalloc(workspace)");
+  }
+  c->ws_bytes = bytes;
+  return TS_OK;
+}
+
+bool valid_dtype(int dt) { return dt >= TS_U8 && dt <= TS_F64; }
+
+// Workspace layouts.  meta | look-back status | arrays.
+size_t sort_ws(size_t n, int kb, bool own_scratch) {
+  size_t b = up(sizeof(ts::SortMeta));
+  if (n > ts::kSmallMax) b += up(ts::status_bytes(n, kb));
+  if (own_scratch && n > ts::kSmallMax) b += up(n * size_t(kb));
+  return b;
+}
+size_t by_key_ws(size_t n, int kb, size_t gather_bytes) {
+  size_t b = up(sizeof(ts::SortMeta));
+  if (n > ts::kSmallMax) b += up(ts::status_bytes(n, kb)) + 2 * up(n * size_t(kb)) + 2 * up(n * 4);
+  b += up(gather_bytes);
+  return b;
+}
+
+void clock_reset(ts_ctx* c) { c->clk.count = 0; }
+
+__global__ void k_iota(uint32_t* out, size_t n) {
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = g; i < n; i += gs) out[i] = uint32_t(i);
+}
+
+ts_status sort_impl(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, hipStream_t s) {
+  const int kb = ts::dtype_bytes(dt), mode = ts::dtype_mode(dt);
+  clock_reset(c);
+  if (n <= 1) return TS_OK;
+  if (n <= ts::kSmallMax) {
+    hipError_t e = ts::small_sort(kb, mode, data, n, nullptr, 0, 0, nullptr, s, &c->clk);
+    return e == hipSuccess ? TS_OK : hip_fail(c, e, "small_sort");
+  }
+  ts_status st = ensure(c, sort_ws(n, kb, buf == nullptr));
+  if (st != TS_OK) return st;
+  char* p = c->ws;
+  ts::RadixJob j;
+  j.key_bytes = kb;
+  j.mode = mode;
+  j.n = n;
+  j.meta = reinterpret_cast<ts::SortMeta*>(p);
+  p += up(sizeof(ts::SortMeta));
+  j.status = p;
+  p += up(ts::status_bytes(n, kb));
+  j.keys_a = data;
+  j.keys_b = buf ? buf : p;
+  j.num_cus = c->num_cus;
+  hipError_t e = ts::radix_sort(j, s, &c->clk);
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ts_status_string(ts_status s) {
+  switch (s) {
+    case TS_OK: return "ok";
+    case TS_ERR_BUFFER_TOO_SMALL: return "buffer too small";
+    case TS_ERR_LENGTH_MISMATCH: return "length mismatch";
+    case TS_ERR_TOO_LONG: return "too long for a u32 order";
+    case TS_ERR_INVALID_ARG: return "invalid argument";
+    case TS_ERR_HIP: return "HIP runtime error";
+    case TS_ERR_VERIFY_FAILED: return "output verification failed";
+    case TS_ERR_NO_DEVICE: return "no such device";
+  }
+  return "unknown status";
+}
+
+ts_status ts_ctx_create(ts_ctx** out, int device) {
+  if (!out) return TS_ERR_INVALID_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return TS_ERR_NO_DEVICE;
+  ts_ctx* c = new ts_ctx();
+  c->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->num_cus = cus;
+  *out = c;
+  return TS_OK;
+}
+
+void ts_ctx_destroy(ts_ctx* c) {
+  if (!c) return;
+  DeviceScope ds(c->device);
+  if (c->ws) hipFree(c->ws);
+  if (c->clk.made)
+    for (int i = 0; i < 2 * ts::KernelClock::kMax; ++i) hipEventDestroy(c->clk.ev[i]);
+  delete c;
+}
+
+ts_status ts_ctx_reserve(ts_ctx* c, ts_dtype dt, size_t n, size_t record_bytes) {
+  if (!c || !valid_dtype(dt)) return TS_ERR_INVALID_ARG;
+  DeviceScope ds(c->device);
+  const int kb = ts::dtype_bytes(dt);
+  size_t need = std::max(sort_ws(n, kb, true), by_key_ws(n, kb, n * record_bytes));
+  return ensure(c, need);
+}
+
+size_t ts_ctx_workspace_bytes(const ts_ctx* c) { return c ? c->ws_bytes : 0; }
+
+const char* ts_ctx_last_error(const ts_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+ts_status ts_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(dt)) return fail(c, TS_ERR_INVALID_ARG, "unknown dtype");
+  if (n > 1 && !data) return fail(c, TS_ERR_INVALID_ARG, "null data");
+  DeviceScope ds(c->device);
+  return sort_impl(c, dt, data, n, nullptr, static_cast<hipStream_t>(stream));
+}
+
+ts_status ts_sort_with_buffer(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, size_t buf_len,
+                              void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(dt)) return fail(c, TS_ERR_INVALID_ARG, "unknown dtype");
+  if (buf_len < n)  // dispatch.rs:78-83
+    return fail(c, TS_ERR_BUFFER_TOO_SMALL,
+                "buffer too small: need " + std::to_string(n) + ", got " + std::to_string(buf_len));
+  if (n > 1 && (!data || !buf)) return fail(c, TS_ERR_INVALID_ARG, "null data or buffer");
+  DeviceScope ds(c->device);
+  return sort_impl(c, dt, data, n, buf, static_cast<hipStream_t>(stream));
+}
+
+ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t stride, size_t key_offset,
+                         uint32_t* order, int gather_records, void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(kt)) return fail(c, TS_ERR_INVALID_ARG, "unknown key dtype");
+  const int kb = ts::dtype_bytes(kt), mode = ts::dtype_mode(kt);
+  if (stride < size_t(kb) || key_offset + size_t(kb) > stride)
+    return fail(c, TS_ERR_INVALID_ARG, "key does not fit inside the record stride");
+  if ((reinterpret_cast<uintptr_t>(records) + key_offset) % size_t(kb) != 0 || stride % size_t(kb) != 0)
+    return fail(c, TS_ERR_INVALID_ARG, "record keys must be aligned to the key width");
+  if (n > size_t(UINT32_MAX))  // by_key.rs:190-194
+    return fail(c, TS_ERR_TOO_LONG, "more than u32::MAX elements: order() stays empty");
+  if (n > 0 && (!records || !order)) return fail(c, TS_ERR_INVALID_ARG, "null records or order");
+  DeviceScope ds(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  clock_reset(c);
+  if (n == 0) return TS_OK;
+  const unsigned char* rec = static_cast<const unsigned char*>(records);
+  const size_t gbytes = gather_records ? n * stride : 0;
+  ts_status st = ensure(c, by_key_ws(n, kb, gbytes));
+  if (st != TS_OK) return st;
+  char* p = c->ws;
+  hipError_t e;
+  if (n < 2) {  // by_key.rs:185-189
+    hipLaunchKernelGGL(k_iota, dim3(1), dim3(64), 0, s, order, n);
+    return TS_OK;
+  }
+  char* gbuf;
+  if (n <= ts::kSmallMax) {
+    gbuf = p + up(sizeof(ts::SortMeta));
+    e = ts::small_sort(kb, mode, nullptr, n, rec, stride, key_offset, order, s, &c->clk);
+    if (e != hipSuccess) return hip_fail(c, e, "small_sort");
+  } else {
+    ts::RadixJob j;
+    j.key_bytes = kb;
+    j.mode = mode;
+    j.n = n;
+    j.records = rec;
+    j.stride = stride;
+    j.key_offset = key_offset;
+    j.meta = reinterpret_cast<ts::SortMeta*>(p);
+    p += up(sizeof(ts::SortMeta));
+    j.status = p;
+    p += up(ts::status_bytes(n, kb));
+    j.keys_a = p;
+    p += up(n * size_t(kb));
+    j.keys_b = p;
+    p += up(n * size_t(kb));
+    j.vals_a = reinterpret_cast<uint32_t*>(p);
+    p += up(n * 4);
+    j.vals_b = reinterpret_cast<uint32_t*>(p);
+    p += up(n * 4);
+    j.final_vals = order;
+    j.num_cus = c->num_cus;
+    gbuf = p;
+    e = ts::radix_sort(j, s, &c->clk);
+    if (e != hipSuccess) return hip_fail(c, e, "radix_sort");
+  }
+  if (gather_records) {  // move_into_order (by_key.rs:315-334): gather, then copy back
+    e = ts::gather(order, records, gbuf, n, stride, s, &c->clk);
+    if (e != hipSuccess) return hip_fail(c, e, "gather");
+    e = hipMemcpyAsync(records, gbuf, n * stride, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpyAsync");
+  }
+  return TS_OK;
+}
+
+ts_status ts_apply_order(ts_ctx* c, const uint32_t* order, size_t order_len, void* other, size_t n,
+                         size_t elem_bytes, void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (order_len != n)  // by_key.rs:137-141
+    return fail(c, TS_ERR_LENGTH_MISMATCH, "apply_order: the slice's length differs from the last sorted slice's");
+  if (elem_bytes == 0) return fail(c, TS_ERR_INVALID_ARG, "zero-sized elements");
+  if (n > 0 && (!order || !other)) return fail(c, TS_ERR_INVALID_ARG, "null order or slice");
+  DeviceScope ds(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  clock_reset(c);
+  if (n == 0) return TS_OK;
+  ts_status st = ensure(c, up(n * elem_bytes));
+  if (st != TS_OK) return st;
+  hipError_t e = ts::gather(order, other, c->ws, n, elem_bytes, s, &c->clk);
+  if (e != hipSuccess) return hip_fail(c, e, "gather");
+  e = hipMemcpyAsync(other, c->ws, n * elem_bytes, hipMemcpyDeviceToDevice, s);
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "hipMemcpyAsync");
+}
+
+ts_status ts_fill_splitmix64(ts_ctx* c, void* out, size_t n, uint64_t seed, int kind, void* stream) {
+  if (!c || kind < 0 || kind > 2 || (n && !out)) return TS_ERR_INVALID_ARG;
+  DeviceScope ds(c->device);
+  hipError_t e = ts::fill_splitmix64(out, n, seed, kind, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "fill_splitmix64");
+}
+
+ts_status ts_checksum(ts_ctx* c, ts_dtype dt, const void* data, size_t n, uint64_t out_sum_xor[2],
+                      uint64_t* out_unsorted, void* stream) {
+  if (!c || !valid_dtype(dt) || !out_sum_xor || !out_unsorted || (n && !data)) return TS_ERR_INVALID_ARG;
+  DeviceScope ds(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  ts_status st = ensure(c, up(sizeof(ts::SortMeta)));
+  if (st != TS_OK) return st;
+  ts::SortMeta* m = reinterpret_cast<ts::SortMeta*>(c->ws);
+  hipError_t e = ts::checksum(ts::dtype_bytes(dt), ts::dtype_mode(dt), data, n, m, s);
+  if (e != hipSuccess) return hip_fail(c, e, "checksum");
+  unsigned long long h[4];
+  e = hipMemcpyAsync(h, m->check, sizeof(h), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(c, e, "checksum readback");
+  out_sum_xor[0] = h[0];
+  out_sum_xor[1] = h[1];
+  *out_unsorted = h[2];
+  return TS_OK;
+}
+
+ts_status ts_ctx_set_profiling(ts_ctx* c, int enable) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  DeviceScope ds(c->device);
+  if (enable && !c->clk.made) {
+    for (int i = 0; i < 2 * ts::KernelClock::kMax; ++i)
+      if (hipEventCreate(&c->clk.ev[i]) != hipSuccess) return fail(c, TS_ERR_HIP, "hipEventCreate");
+    c->clk.made = true;
+  }
+  c->clk.on = enable != 0;
+  c->clk.count = 0;
+  return TS_OK;
+}
+
+int ts_ctx_kernel_times(const ts_ctx* c, const char** names, float* ms, double* bytes, int cap) {
+  if (!c || !c->clk.on) return 0;
+  DeviceScope ds(c->device);
+  const int k = std::min(cap, c->clk.count);
+  for (int i = 0; i < k; ++i) {
+    float t = 0.f;
+    hipEventSynchronize(c->clk.ev[2 * i + 1]);
+    hipEventElapsedTime(&t, c->clk.ev[2 * i], c->clk.ev[2 * i + 1]);
+    if (names) names[i] = c->clk.name[i];
+    if (ms) ms[i] = t;
+    if (bytes) bytes[i] = c->clk.bytes[i];
+  }
+  return k;
+}
+
+}  // extern "C"

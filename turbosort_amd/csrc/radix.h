@@ -1,0 +1,88 @@
+// Host-side entry points of the device radix sort (radix.hip) and the other
+// kernel files, used by the C ABI in api.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "keys.h"
+
+namespace ts {
+
+// Records the duration of each kernel of a sort with HIP events on the sort's
+// stream, when enabled on the context.
+struct KernelClock {
+  static constexpr int kMax = 64;
+  bool on = false;
+  int count = 0;
+  const char* name[kMax];
+  double bytes[kMax];
+  hipEvent_t ev[2 * kMax];
+  bool made = false;
+  void begin(hipStream_t s, const char* nm, double b) {
+    if (!on || count >= kMax) return;
+    name[count] = nm;
+    bytes[count] = b;
+    hipEventRecord(ev[2 * count], s);
+  }
+  void end(hipStream_t s) {
+    if (!on || count >= kMax) return;
+    hipEventRecord(ev[2 * count + 1], s);
+    ++count;
+  }
+};
+
+// Tile geometry of the one-sweep scatter: 256 lanes (4 wave64s) per workgroup.
+constexpr int kSweepThreads = 256;
+template <int KB> struct SweepShape {
+  static constexpr int ipt = 16;  // keys per lane
+  static constexpr int tile = kSweepThreads * ipt;
+};
+
+inline uint32_t sweep_tiles(size_t n, int key_bytes) {
+  (void)key_bytes;
+  const size_t tile = size_t(kSweepThreads) * SweepShape<4>::ipt;
+  return uint32_t((n + tile - 1) / tile);
+}
+
+// Look-back words are 32-bit (2 flag bits, 30 value bits) while every count
+// fits in 30 bits, else 64-bit.
+inline bool wide_status(size_t n) { return n >= (size_t(1) << 30); }
+inline size_t status_bytes(size_t n, int key_bytes) {
+  return size_t(2) * sweep_tiles(n, key_bytes) * 256 * (wide_status(n) ? 8 : 4);
+}
+
+// One LSD sort on the device.  Keys start in keys_a (plain sort) or are read
+// from records (by-key sort; keys_a then receives the transformed keys).
+struct RadixJob {
+  int key_bytes = 4;
+  int mode = kIdentity;  // transform of the source keys
+  size_t n = 0;
+  const unsigned char* records = nullptr;  // by-key: record base
+  size_t stride = 0, key_offset = 0;
+  void* keys_a = nullptr;
+  void* keys_b = nullptr;
+  uint32_t* vals_a = nullptr;  // by-key: order ping-pong
+  uint32_t* vals_b = nullptr;
+  uint32_t* final_vals = nullptr;  // by-key: the last pass writes order here
+  SortMeta* meta = nullptr;
+  void* status = nullptr;
+  int num_cus = 256;
+};
+
+hipError_t radix_sort(const RadixJob& job, hipStream_t s, KernelClock* clk);
+
+// Small-input tier: one workgroup sorts n <= kSmallMax keys in LDS with a
+// bitonic network.  With order != nullptr it sorts (key, position) pairs and
+// writes the stable order instead of moving keys (by-key path).
+constexpr size_t kSmallMax = 4096;
+hipError_t small_sort(int key_bytes, int mode, void* keys, size_t n, const unsigned char* records, size_t stride,
+                      size_t key_offset, uint32_t* order, hipStream_t s, KernelClock* clk);
+
+// out[i] = src[order[i]] for n elements of elem_bytes bytes.
+hipError_t gather(const uint32_t* order, const void* src, void* dst, size_t n, size_t elem_bytes, hipStream_t s,
+                  KernelClock* clk);
+
+hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStream_t s);
+hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, SortMeta* meta, hipStream_t s);
+
+}  // namespace ts

@@ -1,0 +1,155 @@
+// Element gather (move_into_order / apply_order), synthetic key generation
+// and the order-independent output check.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "radix.h"
+
+namespace ts {
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned grid_for(size_t n, size_t per_thread = 1) {
+  size_t b = (n + kThreads * per_thread - 1) / (kThreads * per_thread);
+  return unsigned(std::max<size_t>(1, std::min<size_t>(b, 65536)));
+}
+
+// by_key.rs:315-334: out[i] = src[order[i]].  Coalesced order read and
+// output write; the source read is random.
+template <class E>
+__global__ void k_gather(const uint32_t* __restrict__ order, const E* __restrict__ src, E* __restrict__ dst,
+                         size_t n) {
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = g; i < n; i += gs) dst[i] = src[order[i]];
+}
+
+__global__ void k_gather_bytes(const uint32_t* __restrict__ order, const unsigned char* __restrict__ src,
+                               unsigned char* __restrict__ dst, size_t n, size_t eb) {
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = g; i < n; i += gs) {
+    const unsigned char* s = src + size_t(order[i]) * eb;
+    unsigned char* d = dst + i * eb;
+    for (size_t b = 0; b < eb; ++b) d[b] = s[b];
+  }
+}
+
+__global__ void k_gather_words(const uint32_t* __restrict__ order, const uint32_t* __restrict__ src,
+                               uint32_t* __restrict__ dst, size_t n, size_t words) {
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = g; i < n; i += gs) {
+    const uint32_t* s = src + size_t(order[i]) * words;
+    uint32_t* d = dst + i * words;
+    for (size_t b = 0; b < words; ++b) d[b] = s[b];
+  }
+}
+
+__global__ void k_splitmix(void* out, size_t n, uint64_t seed, int kind) {
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = g; i < n; i += gs) {
+    const uint64_t z = splitmix64_at(seed, i);
+    if (kind == 0)
+      static_cast<uint32_t*>(out)[i] = uint32_t(z >> 32);
+    else if (kind == 1)
+      static_cast<uint64_t*>(out)[i] = z;
+    else
+      static_cast<uint64_t*>(out)[i] = z >> 32;
+  }
+}
+
+__global__ void k_check_zero(SortMeta* m) {
+  if (threadIdx.x < 4) m->check[threadIdx.x] = 0;
+}
+
+// Sum and xor of mix64(bits) over the keys (a multiset fingerprint that any
+// permutation preserves) and the number of adjacent pairs out of order.
+template <class U>
+__global__ void k_check(const U* __restrict__ x, size_t n, int mode, SortMeta* m) {
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  unsigned long long sum = 0, xr = 0, bad = 0;
+  for (size_t i = g; i < n; i += gs) {
+    const U a = x[i];
+    const uint64_t h = mix64(uint64_t(a) + 0x632BE59BD9B4E019ull);
+    sum += h;
+    xr ^= h;
+    if (i + 1 < n && to_key(a, mode) > to_key(x[i + 1], mode)) ++bad;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    sum += __shfl_xor(sum, d, 64);
+    xr ^= __shfl_xor(xr, d, 64);
+    bad += __shfl_xor(bad, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&m->check[0], sum);
+    atomicXor(&m->check[1], xr);
+    atomicAdd(&m->check[2], bad);
+  }
+}
+
+}  // namespace
+
+hipError_t gather(const uint32_t* order, const void* src, void* dst, size_t n, size_t eb, hipStream_t s,
+                  KernelClock* clk) {
+  if (n == 0) return hipSuccess;
+  const uintptr_t al = reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst);
+  const unsigned grid = grid_for(n);
+  if (clk) clk->begin(s, "k_gather", double(n) * (4.0 + 2.0 * eb));
+  if (eb == 16 && al % 16 == 0)
+    hipLaunchKernelGGL(k_gather<uint4>, dim3(grid), dim3(kThreads), 0, s, order, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n);
+  else if (eb == 8 && al % 8 == 0)
+    hipLaunchKernelGGL(k_gather<uint64_t>, dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const uint64_t*>(src), static_cast<uint64_t*>(dst), n);
+  else if (eb == 4 && al % 4 == 0)
+    hipLaunchKernelGGL(k_gather<uint32_t>, dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), n);
+  else if (eb == 2 && al % 2 == 0)
+    hipLaunchKernelGGL(k_gather<uint16_t>, dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), n);
+  else if (eb == 1)
+    hipLaunchKernelGGL(k_gather<uint8_t>, dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const uint8_t*>(src), static_cast<uint8_t*>(dst), n);
+  else if (eb % 4 == 0 && al % 4 == 0)
+    hipLaunchKernelGGL(k_gather_words, dim3(grid), dim3(kThreads), 0, s, order, static_cast<const uint32_t*>(src),
+                       static_cast<uint32_t*>(dst), n, eb / 4);
+  else
+    hipLaunchKernelGGL(k_gather_bytes, dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const unsigned char*>(src), static_cast<unsigned char*>(dst), n, eb);
+  if (clk) clk->end(s);
+  return hipGetLastError();
+}
+
+hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_splitmix, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, out, n, seed, kind);
+  return hipGetLastError();
+}
+
+hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, SortMeta* meta, hipStream_t s) {
+  hipLaunchKernelGGL(k_check_zero, dim3(1), dim3(64), 0, s, meta);
+  if (n == 0) return hipGetLastError();
+  const unsigned grid = grid_for(n, 8);
+  switch (key_bytes) {
+    case 1:
+      hipLaunchKernelGGL(k_check<uint8_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint8_t*>(data), n,
+                         mode, meta);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_check<uint16_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint16_t*>(data), n,
+                         mode, meta);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_check<uint32_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint32_t*>(data), n,
+                         mode, meta);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_check<uint64_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint64_t*>(data), n,
+                         mode, meta);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ts
