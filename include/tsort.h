@@ -83,6 +83,22 @@ ts_status ts_sort_by_key(ts_ctx* ctx, ts_dtype key_type, void* d_records, size_t
 ts_status ts_apply_order(ts_ctx* ctx, const uint32_t* d_order, size_t order_len, void* d_other, size_t n,
                          size_t elem_bytes, void* stream);
 
+/* ---- SortByKeyBuffer (by_key.rs:51-159) as an object ----
+ * A context plus a grow-only device order array.  ts_sbk_sort is
+ * sort_by_key_with_buffer (lib.rs:196-202); ts_sbk_order is order()
+ * (by_key.rs:108-110), empty (len 0) before the first sort; ts_sbk_apply_order
+ * is apply_order (by_key.rs:136-144); ts_sbk_capacity is capacity()
+ * (by_key.rs:84-90). */
+typedef struct ts_sbk ts_sbk;
+ts_status ts_sbk_create(ts_sbk** out, int device);
+void ts_sbk_destroy(ts_sbk* b);
+size_t ts_sbk_capacity(const ts_sbk* b);
+const uint32_t* ts_sbk_order(const ts_sbk* b, size_t* len);
+ts_status ts_sbk_sort(ts_sbk* b, ts_dtype key_type, void* d_records, size_t n, size_t stride, size_t key_offset,
+                      int gather_records, void* stream);
+ts_status ts_sbk_apply_order(ts_sbk* b, void* d_other, size_t n, size_t elem_bytes, void* stream);
+const char* ts_sbk_last_error(const ts_sbk* b);
+
 /* ---- Utilities for tests and benchmarks (no reference counterpart) ---- */
 
 /* Fill d_out with n synthetic keys from SplitMix64 (Vigna), element i taking

@@ -16,6 +16,8 @@ EXPORTS = [
     "ts_ctx_create", "ts_ctx_destroy", "ts_ctx_reserve", "ts_ctx_workspace_bytes", "ts_ctx_last_error",
     "ts_status_string", "ts_sort", "ts_sort_with_buffer", "ts_sort_by_key", "ts_apply_order",
     "ts_fill_splitmix64", "ts_checksum", "ts_ctx_set_profiling", "ts_ctx_kernel_times",
+    "ts_sbk_create", "ts_sbk_destroy", "ts_sbk_capacity", "ts_sbk_order", "ts_sbk_sort", "ts_sbk_apply_order",
+    "ts_sbk_last_error",
 ]
 
 TS_OK = 0
@@ -70,5 +72,19 @@ def load() -> ctypes.CDLL:
     L.ts_ctx_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_double), i32]
     L.ts_ctx_kernel_times.restype = i32
+    L.ts_sbk_create.argtypes = [ctypes.POINTER(vp), i32]
+    L.ts_sbk_create.restype = i32
+    L.ts_sbk_destroy.argtypes = [vp]
+    L.ts_sbk_destroy.restype = None
+    L.ts_sbk_capacity.argtypes = [vp]
+    L.ts_sbk_capacity.restype = sz
+    L.ts_sbk_order.argtypes = [vp, ctypes.POINTER(sz)]
+    L.ts_sbk_order.restype = vp
+    L.ts_sbk_sort.argtypes = [vp, i32, vp, sz, sz, sz, i32, vp]
+    L.ts_sbk_sort.restype = i32
+    L.ts_sbk_apply_order.argtypes = [vp, vp, sz, sz, vp]
+    L.ts_sbk_apply_order.restype = i32
+    L.ts_sbk_last_error.argtypes = [vp]
+    L.ts_sbk_last_error.restype = ctypes.c_char_p
     _lib = L
     return L

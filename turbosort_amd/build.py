@@ -57,6 +57,21 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+CPP_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "hpp_test.cpp")
+CPP_TEST_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "hpp_test.bin")
+
+
+def build_cpp_test() -> str:
+    """The C++ API test program (include/turbosort.hpp over libtsort.so)."""
+    cmd = [HIPCC, *FLAGS, CPP_TEST_SRC, "-o", CPP_TEST_BIN, "-L", HERE, "-ltsort",
+           f"-Wl,-rpath,{os.path.relpath(HERE, os.path.dirname(CPP_TEST_BIN))}".replace("-rpath,", "-rpath,$ORIGIN/")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on hpp_test.cpp:\n{r.stdout}\n{r.stderr}")
+    return CPP_TEST_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_test()
     sys.exit(0)

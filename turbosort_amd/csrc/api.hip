@@ -275,6 +275,77 @@ ts_status ts_apply_order(ts_ctx* c, const uint32_t* order, size_t order_len, voi
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "hipMemcpyAsync");
 }
 
+}  // extern "C"
+
+struct ts_sbk {
+  ts_ctx* ctx = nullptr;
+  uint32_t* order = nullptr;
+  size_t cap = 0, len = 0;
+};
+
+extern "C" {
+
+ts_status ts_sbk_create(ts_sbk** out, int device) {
+  if (!out) return TS_ERR_INVALID_ARG;
+  *out = nullptr;
+  ts_ctx* c = nullptr;
+  ts_status st = ts_ctx_create(&c, device);
+  if (st != TS_OK) return st;
+  ts_sbk* b = new ts_sbk();
+  b->ctx = c;
+  *out = b;
+  return TS_OK;
+}
+
+void ts_sbk_destroy(ts_sbk* b) {
+  if (!b) return;
+  {
+    DeviceScope ds(b->ctx->device);
+    if (b->order) hipFree(b->order);
+  }
+  ts_ctx_destroy(b->ctx);
+  delete b;
+}
+
+size_t ts_sbk_capacity(const ts_sbk* b) { return b ? b->cap : 0; }
+
+const uint32_t* ts_sbk_order(const ts_sbk* b, size_t* len) {
+  if (len) *len = b ? b->len : 0;
+  return b ? b->order : nullptr;
+}
+
+const char* ts_sbk_last_error(const ts_sbk* b) { return b ? ts_ctx_last_error(b->ctx) : "null buffer"; }
+
+ts_status ts_sbk_sort(ts_sbk* b, ts_dtype kt, void* records, size_t n, size_t stride, size_t key_offset,
+                      int gather_records, void* stream) {
+  if (!b) return TS_ERR_INVALID_ARG;
+  if (n > size_t(UINT32_MAX))
+    return fail(b->ctx, TS_ERR_TOO_LONG, "more than u32::MAX elements: order() stays empty");
+  if (n > b->cap) {
+    DeviceScope ds(b->ctx->device);
+    if (b->order) {
+      hipError_t e = hipFree(b->order);
+      b->order = nullptr;
+      b->cap = 0;
+      if (e != hipSuccess) return hip_fail(b->ctx, e, "hipFree(order)");
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->order), n * sizeof(uint32_t));
+    if (e != hipSuccess) {
+      b->order = nullptr;
+      return hip_fail(b->ctx, e, "hipMalloc(order)");
+    }
+    b->cap = n;
+  }
+  ts_status st = ts_sort_by_key(b->ctx, kt, records, n, stride, key_offset, b->order, gather_records, stream);
+  b->len = st == TS_OK ? n : 0;
+  return st;
+}
+
+ts_status ts_sbk_apply_order(ts_sbk* b, void* other, size_t n, size_t elem_bytes, void* stream) {
+  if (!b) return TS_ERR_INVALID_ARG;
+  return ts_apply_order(b->ctx, b->order, b->len, other, n, elem_bytes, stream);
+}
+
 ts_status ts_fill_splitmix64(ts_ctx* c, void* out, size_t n, uint64_t seed, int kind, void* stream) {
   if (!c || kind < 0 || kind > 2 || (n && !out)) return TS_ERR_INVALID_ARG;
   DeviceScope ds(c->device);

@@ -31,16 +31,31 @@ struct KernelClock {
   }
 };
 
-// Tile geometry of the one-sweep scatter: 256 lanes (4 wave64s) per workgroup.
+// Tile geometry of the one-sweep scatter: 256 lanes (4 wave64s) per
+// workgroup, `ipt` keys per lane.
+#ifndef TS_SWEEP_IPT_NARROW
+#define TS_SWEEP_IPT_NARROW 32
+#endif
+#ifndef TS_SWEEP_IPT_WIDE
+#define TS_SWEEP_IPT_WIDE 16
+#endif
+// Predecessor tiles one look-back step reads at once.
+#ifndef TS_LOOKBACK_WINDOW
+#define TS_LOOKBACK_WINDOW 8
+#endif
+// Publish a tile's digit counts from an LDS histogram before ranking (1) or
+// from the ranking's own counts after it (0).
+#ifndef TS_EARLY_PUBLISH
+#define TS_EARLY_PUBLISH 1
+#endif
 constexpr int kSweepThreads = 256;
 template <int KB> struct SweepShape {
-  static constexpr int ipt = 16;  // keys per lane
+  static constexpr int ipt = KB >= 8 ? TS_SWEEP_IPT_WIDE : TS_SWEEP_IPT_NARROW;
   static constexpr int tile = kSweepThreads * ipt;
 };
 
 inline uint32_t sweep_tiles(size_t n, int key_bytes) {
-  (void)key_bytes;
-  const size_t tile = size_t(kSweepThreads) * SweepShape<4>::ipt;
+  const size_t tile = key_bytes >= 8 ? size_t(SweepShape<8>::tile) : size_t(SweepShape<4>::tile);
   return uint32_t((n + tile - 1) / tile);
 }
 

@@ -6,6 +6,11 @@
 // LSD, live-pass skipping), histogram.rs:15-45 (all digits in one read),
 // prefix_sum.rs:17-24 (exclusive scan), scatter.rs:26-40 (stable scatter by
 // one digit).  The output is the unique ascending order of the radix keys.
+//
+// Code-generation notes for gfx950.  Loads are issued unconditionally (the
+// last tile clamps its indices and masks the extra lanes afterwards) so that
+// all of a lane's loads are in flight together; the key transform is applied
+// in a separate loop under a uniform branch.  In-tile indices are 32-bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,12 +19,6 @@
 
 namespace ts {
 namespace {
-
-template <int KB> struct UOf;
-template <> struct UOf<1> { using T = uint8_t; };
-template <> struct UOf<2> { using T = uint16_t; };
-template <> struct UOf<4> { using T = uint32_t; };
-template <> struct UOf<8> { using T = uint64_t; };
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
@@ -45,6 +44,26 @@ template <class T> __device__ __forceinline__ T block256_exclusive_add(T v, T* s
   return off + inc - v;
 }
 
+template <class U> __device__ __forceinline__ U first_lane(U x) {
+  if constexpr (sizeof(U) == 8) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(x) >> 32));
+    return U((uint64_t(hi) << 32) | lo);
+  } else {
+    return U(__builtin_amdgcn_readfirstlane(uint32_t(x)));
+  }
+}
+
+template <class U, int N> __device__ __forceinline__ void to_keys(U (&k)[N], int mode) {
+  if (mode == kSigned) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) k[j] = to_key(k[j], kSigned);
+  } else if (mode == kFloat) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) k[j] = to_key(k[j], kFloat);
+  }
+}
+
 // ------------------------------------------------------------------ init
 __global__ void k_init(SortMeta* m) {
   for (unsigned i = threadIdx.x; i < 8 * 256; i += blockDim.x) (&m->hist[0][0])[i] = 0;
@@ -59,20 +78,26 @@ __global__ void k_init(SortMeta* m) {
 // All P digit histograms in one read of the keys (histogram.rs:15-45).  Each
 // workgroup counts a contiguous chunk into LDS and adds its counts to the
 // global histogram once.  The forward key transform is fused into the load;
-// the by-key path reads keys out of records and writes the transformed key
-// array the scatter passes start from.  The grid also clears the first
-// look-back row.
+// the by-key path (REC) reads keys out of records and writes the transformed
+// key array the scatter passes start from.
+//
+// A digit on which all of a wave's keys of one step agree (constant digits,
+// sorted runs) is counted with one LDS add instead of 64 colliding atomics.
+// The grid also clears the first look-back row.
 constexpr int kHistThreads = 256;
-constexpr int kHistUnroll = 8;
+template <class U> struct HistUnroll {
+  static constexpr int value = sizeof(U) >= 8 ? 8 : 16;
+};
 
-template <class U>
+template <class U, bool REC>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ keys, const unsigned char* __restrict__ rec,
                                                        size_t stride, size_t key_off, U* __restrict__ keys_out,
                                                        size_t n, int mode, size_t per_block, SortMeta* m,
                                                        uint32_t* zero_ptr, size_t zero_words) {
   constexpr int P = int(sizeof(U));
+  constexpr int UN = HistUnroll<U>::value;
   __shared__ uint32_t h[P * 256];
-  const unsigned tid = threadIdx.x;
+  const unsigned tid = threadIdx.x, lane = tid & 63;
   for (unsigned i = tid; i < P * 256; i += kHistThreads) h[i] = 0;
   {
     const size_t g = size_t(blockIdx.x) * kHistThreads + tid, gs = size_t(gridDim.x) * kHistThreads;
@@ -81,22 +106,50 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ key
   __syncthreads();
   const size_t begin = size_t(blockIdx.x) * per_block;
   const size_t end = begin + per_block < n ? begin + per_block : n;
-  for (size_t base = begin; base < end; base += size_t(kHistThreads) * kHistUnroll) {
-    U k[kHistUnroll];
+  for (size_t base = begin; base < end; base += size_t(kHistThreads) * UN) {
+    const bool full = base + size_t(kHistThreads) * UN <= end;
+    U k[UN];
 #pragma unroll
-    for (int r = 0; r < kHistUnroll; ++r) {
+    for (int r = 0; r < UN; ++r) {
+      size_t i = base + size_t(r) * kHistThreads + tid;
+      if (!full) i = i < end ? i : end - 1;
+      k[r] = REC ? *reinterpret_cast<const U*>(rec + i * stride + key_off) : keys[i];
+    }
+    to_keys(k, mode);
+    if (REC) {
+#pragma unroll
+      for (int r = 0; r < UN; ++r) {
+        const size_t i = base + size_t(r) * kHistThreads + tid;
+        if (full || i < end) keys_out[i] = k[r];
+      }
+    }
+    // Bits in which the wave's valid keys differ from its first key.
+    const U k0 = first_lane(k[0]);
+    U diff = 0;
+    uint32_t wave_count = 64u * UN;
+#pragma unroll
+    for (int r = 0; r < UN; ++r) {
       const size_t i = base + size_t(r) * kHistThreads + tid;
-      U x = 0;
-      if (i < end) x = rec ? *reinterpret_cast<const U*>(rec + i * stride + key_off) : keys[i];
-      k[r] = to_key(x, mode);
+      const bool ok = full || i < end;
+      if (ok) diff |= U(k[r] ^ k0);
+    }
+    if (!full) {
+      wave_count = 0;
+#pragma unroll
+      for (int r = 0; r < UN; ++r)
+        wave_count += uint32_t(__popcll(__ballot(base + size_t(r) * kHistThreads + tid < end)));
     }
 #pragma unroll
-    for (int r = 0; r < kHistUnroll; ++r) {
-      const size_t i = base + size_t(r) * kHistThreads + tid;
-      if (i < end) {
-        if (keys_out) keys_out[i] = k[r];
+    for (int p = 0; p < P; ++p) {
+      const bool varies = unsigned((uint64_t(diff) >> (8 * p)) & 0xFF) != 0;
+      if (__ballot(varies) == 0) {
+        if (lane == 0) atomicAdd(&h[p * 256 + unsigned((uint64_t(k0) >> (8 * p)) & 0xFF)], wave_count);
+      } else {
 #pragma unroll
-        for (int p = 0; p < P; ++p) atomicAdd(&h[p * 256 + unsigned((uint64_t(k[r]) >> (8 * p)) & 0xFF)], 1u);
+        for (int r = 0; r < UN; ++r) {
+          const size_t i = base + size_t(r) * kHistThreads + tid;
+          if (full || i < end) atomicAdd(&h[p * 256 + unsigned((uint64_t(k[r]) >> (8 * p)) & 0xFF)], 1u);
+        }
       }
     }
   }
@@ -155,15 +208,79 @@ template <class U> struct SweepArgs {
 // About two seconds of s_sleep(1) polling.
 constexpr uint32_t kSpinLimit = 1u << 26;
 
+// The byte digit of a key at bit `shift` (a multiple of 8, uniform).
+template <class U> __device__ __forceinline__ uint32_t digit_of(U k, unsigned shift) {
+  if constexpr (sizeof(U) == 8) {
+    const uint32_t part = (shift & 32u) ? uint32_t(uint64_t(k) >> 32) : uint32_t(k);
+    return __builtin_amdgcn_ubfe(part, shift & 31u, 8u);
+  } else {
+    return __builtin_amdgcn_ubfe(uint32_t(k), shift, 8u);
+  }
+}
+
+// x & ~(vote ^ m): keep the lanes whose vote equals this lane's bit, as one
+// v_bitop3_b32 (truth table 0x90 over x, vote, m).  A builtin, not inline
+// asm, so the compiler schedules the wait states a VALU read of the freshly
+// written vote (an SGPR) needs.
+__device__ __forceinline__ uint32_t keep_equal(uint32_t x, uint32_t vote, uint32_t m) {
+  return __builtin_amdgcn_bitop3_b32(x, vote, m, 0x90);
+}
+
+// Stable rank of each of a lane's IPT keys among the keys of its wave that
+// carry the same digit, counted into the wave's LDS histogram.  Lanes with
+// equal digits are found with eight ballots: lane l keeps the lanes whose
+// vote on every digit bit equals its own, i.e. ~(vote ^ mybit) per bit.
+template <class U, int IPT, bool FULL>
+__device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[IPT], unsigned shift,
+                                             uint32_t* whist, uint32_t lb, uint32_t valid) {
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t d = digit_of(k[j], shift);
+    uint32_t lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
+    bool ok = true;
+    if (!FULL) {
+      ok = lb + uint32_t(j) * 64 < valid;
+      const uint64_t okm = __ballot(ok);
+      lo = uint32_t(okm);
+      hi = uint32_t(okm >> 32);
+    }
+    // All eight votes first, then the combines: a vote (an SGPR) read by the
+    // VALU right after the compare that wrote it costs wait states.
+    uint32_t m[8];
+    uint64_t vote[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      m[b] = uint32_t(__builtin_amdgcn_sbfe(int32_t(d), uint32_t(b), 1u));  // 0 or ~0
+      // Emits nothing; keeps the compiler from re-deriving the vote from d
+      // with a second shift instead of comparing m.
+      asm("" : "+v"(m[b]));
+      vote[b] = __ballot(m[b] != 0);
+    }
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      lo = keep_equal(lo, uint32_t(vote[b]), m[b]);
+      hi = keep_equal(hi, uint32_t(vote[b] >> 32), m[b]);
+    }
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    const uint32_t before = whist[d];
+    // Every lane of a digit group writes the same new count, so no branch is
+    // needed.  A masked-off lane of the last tile holds a copy of a real key;
+    // its group holds only the valid lanes sharing that digit, so it writes
+    // their count (or the unchanged count when there are none).
+    whist[d] = before + uint32_t(__popc(lo) + __popc(hi));
+    rk[j] = before + below;
+    (void)ok;
+  }
+}
+
 // One stable scatter pass by the byte digit meta->live[pass] (scatter.rs:26-40).
 //
 // Tiles are claimed in launch order through a ticket, so every tile a tile
-// waits on has already started.  Each wave ranks its 16x64 keys stably with
-// eight ballots per key (lanes agreeing on all digit bits), counting into a
-// wave-private LDS histogram; a cross-wave scan then gives tile-local ranks.
-// The tile's digit counts are published, the exclusive prefix over earlier
-// tiles is gathered by decoupled look-back, and the tile is staged in LDS in
-// digit order so that each digit's keys leave as one coalesced run.
+// waits on has already started.  Each wave ranks its IPT x 64 keys stably
+// (rank_in_wave); a cross-wave scan gives tile-local positions.  The tile's
+// digit counts are published, the exclusive prefix over earlier tiles is
+// gathered by decoupled look-back, and the tile is staged in LDS in digit
+// order so that each digit's keys leave as one coalesced run.
 //
 // Pass 0 of a by-key sort generates the values (positions 0..n-1); the last
 // pass applies the inverse key transform, and in a by-key sort writes only the
@@ -179,10 +296,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
   __shared__ uint32_t s_whist[NW][256];
   __shared__ U s_keys[TILE];
   __shared__ uint32_t s_vals[VALS ? TILE : 1];
-  __shared__ uint32_t s_excl[256];
   __shared__ unsigned long long s_gofs[256];
   __shared__ uint32_t s_wave[NW];
   __shared__ uint32_t s_tile;
+#if TS_EARLY_PUBLISH
+  __shared__ uint32_t s_tcnt[256];
+#endif
 
   const uint32_t n_live = a.meta->n_live;
   if (uint32_t(pass) >= n_live) return;
@@ -199,6 +318,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
   const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (tid == 0) s_tile = atomicAdd(&a.tickets[pass], 1u);
   for (unsigned i = tid; i < NW * 256; i += kSweepThreads) (&s_whist[0][0])[i] = 0;
+#if TS_EARLY_PUBLISH
+  s_tcnt[tid] = 0;
+#endif
   __syncthreads();
   const uint32_t tile = s_tile;
   S* row = status + size_t(pass & 1) * a.num_tiles * 256;
@@ -207,72 +329,113 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
 
   const size_t n = a.n;
   const size_t tile_base = size_t(tile) * TILE;
-  const size_t wbase = tile_base + size_t(w) * 64 * IPT;
+  const uint32_t valid = n - tile_base >= size_t(TILE) ? uint32_t(TILE) : uint32_t(n - tile_base);
+  const bool full = valid == uint32_t(TILE);
+  const uint32_t lb = w * (64 * IPT) + lane;  // this lane's first key in the tile
 
   U k[IPT];
-  uint32_t v[IPT];
+  uint32_t v[VALS ? IPT : 1];
+  {
+    const U* __restrict__ ts = src + tile_base;
+    if (full) {
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const size_t i = wbase + size_t(j) * 64 + lane;
-    k[j] = i < n ? (first ? to_key(src[i], a.mode_in) : src[i]) : U(~U(0));
-    if (VALS) v[j] = first ? uint32_t(i) : (i < n ? vsrc[i] : 0u);
+      for (int j = 0; j < IPT; ++j) k[j] = ts[lb + uint32_t(j) * 64];
+    } else {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t li = lb + uint32_t(j) * 64;
+        k[j] = ts[li < valid ? li : valid - 1];
+      }
+    }
+    if (VALS) {
+      if (first) {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) v[j] = uint32_t(tile_base) + lb + uint32_t(j) * 64;
+      } else {
+        const uint32_t* __restrict__ tv = vsrc + tile_base;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          const uint32_t li = lb + uint32_t(j) * 64;
+          v[j] = tv[full || li < valid ? li : valid - 1];
+        }
+      }
+    }
+    if (first) to_keys(k, a.mode_in);
   }
 
-  // Stable rank of each key among the wave's keys with the same digit.
+  S* my = row + size_t(tile) * 256 + tid;
+#if TS_EARLY_PUBLISH
+  // Publish the tile's digit counts before ranking, so that later tiles
+  // rarely find this one unpublished.
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (full || lb + uint32_t(j) * 64 < valid) atomicAdd(&s_tcnt[digit_of(k[j], shift)], 1u);
+  __syncthreads();
+  __hip_atomic_store(my, S(s_tcnt[tid]) | (tile == 0 ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+
   uint32_t rk[IPT];
-  const uint64_t lt_mask = (1ull << lane) - 1;
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const bool ok = wbase + size_t(j) * 64 + lane < n;
-    const unsigned d = unsigned(uint64_t(k[j]) >> shift) & 0xFF;
-    uint64_t same = __ballot(ok);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1;
-      const uint64_t vote = __ballot(bit);
-      same &= bit ? vote : ~vote;
-    }
-    const uint64_t below = same & lt_mask;
-    const uint32_t before = s_whist[w][d];
-    if (ok && below == 0) s_whist[w][d] = before + uint32_t(__popcll(same));
-    rk[j] = before + uint32_t(__popcll(below));
-  }
+  if (full)
+    rank_in_wave<U, IPT, true>(k, rk, shift, s_whist[w], lb, valid);
+  else
+    rank_in_wave<U, IPT, false>(k, rk, shift, s_whist[w], lb, valid);
   __syncthreads();
 
   // Per digit: exclusive offsets of the waves, and the tile's count.
   uint32_t cnt = 0;
+  uint32_t wexcl[NW];
+#pragma unroll
   for (int ww = 0; ww < NW; ++ww) {
-    const uint32_t c = s_whist[ww][tid];
-    s_whist[ww][tid] = cnt;
-    cnt += c;
+    wexcl[ww] = cnt;
+    cnt += s_whist[ww][tid];
   }
-  S* my = row + size_t(tile) * 256 + tid;
+#if !TS_EARLY_PUBLISH
   __hip_atomic_store(my, S(cnt) | (tile == 0 ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
   const uint32_t excl_in_tile = block256_exclusive_add(cnt, s_wave);
-  s_excl[tid] = excl_in_tile;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) s_whist[ww][tid] = excl_in_tile + wexcl[ww];
 
-  // Decoupled look-back over earlier tiles for this lane's digit.
-  // Every tile with a smaller ticket is running and publishes without waiting
-  // on later tiles, so the walk ends; the spin bound only turns a broken
-  // invariant into a flagged wrong result instead of a hung GPU.
+  // Decoupled look-back over earlier tiles for this lane's digit.  Every tile
+  // with a smaller ticket is running and publishes without waiting on later
+  // tiles, so the walk ends; the spin bound only turns a broken invariant
+  // into a flagged wrong result instead of a hung GPU.
+  // Each step reads a window of kWindow predecessors at once and consumes
+  // them newest first until one holds an inclusive prefix.
   unsigned long long before_tiles = 0;
   if (tile > 0) {
+    constexpr int kWindow = TS_LOOKBACK_WINDOW;
     int64_t j = int64_t(tile) - 1;
     uint32_t spins = 0;
     while (true) {
-      const S st = __hip_atomic_load(row + size_t(j) * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const S fl = st >> VB;
-      if (fl == 0) {
+      S st[kWindow];
+#pragma unroll
+      for (int q = 0; q < kWindow; ++q)
+        st[q] = j - q >= 0 ? __hip_atomic_load(row + size_t(j - q) * 256 + tid, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : S(0);
+      int used = 0;
+      bool done = false;
+#pragma unroll
+      for (int q = 0; q < kWindow; ++q) {
+        const S fl = st[q] >> VB;
+        if (fl == 0) break;  // not published yet: wait for it
+        before_tiles += (unsigned long long)(st[q] & kMask);
+        ++used;
+        if (fl == 2 || j - q == 0) {
+          done = true;
+          break;
+        }
+      }
+      if (done) break;
+      j -= used;
+      if (used == 0) {
         if (++spins > kSpinLimit) {
           atomicOr(a.error, 1u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      before_tiles += (unsigned long long)(st & kMask);
-      if (fl == 2 || j == 0) break;
-      --j;
     }
     __hip_atomic_store(my, S(before_tiles + cnt) | kPre, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -282,10 +445,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
   // Stage the tile in digit order.
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const bool ok = wbase + size_t(j) * 64 + lane < n;
-    if (ok) {
-      const unsigned d = unsigned(uint64_t(k[j]) >> shift) & 0xFF;
-      const uint32_t pos = s_excl[d] + s_whist[w][d] + rk[j];
+    if (full || lb + uint32_t(j) * 64 < valid) {
+      const uint32_t d = digit_of(k[j], shift);
+      const uint32_t pos = s_whist[w][d] + rk[j];
       s_keys[pos] = k[j];
       if (VALS) s_vals[pos] = v[j];
     }
@@ -293,16 +455,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
   __syncthreads();
 
   // Write each digit's run of the tile contiguously.
-  const uint32_t valid = n - tile_base < size_t(TILE) ? uint32_t(n - tile_base) : uint32_t(TILE);
   const int out_mode = last ? a.mode_out : int(kIdentity);
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t s = uint32_t(j) * kSweepThreads + tid;
-    if (s < valid) {
+    if (full || s < valid) {
       const U kk = s_keys[s];
-      const unsigned d = unsigned(uint64_t(kk) >> shift) & 0xFF;
+      const uint32_t d = digit_of(kk, shift);
       const size_t o = size_t(s_gofs[d]) + s;
-      if (write_keys) dst[o] = from_key(kk, out_mode);
+      if (write_keys) dst[o] = out_mode == kIdentity ? kk : from_key(kk, out_mode);
       if (VALS) vdst[o] = s_vals[s];
     }
   }
@@ -355,7 +516,7 @@ template <class U> hipError_t run_radix(const RadixJob& j, hipStream_t s, Kernel
   constexpr int P = int(sizeof(U));
   hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, s, j.meta);
   // Histogram: a few workgroups per CU, each over a contiguous chunk.
-  const size_t step = size_t(kHistThreads) * kHistUnroll;
+  const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
   const size_t want_blocks = size_t(j.num_cus) * 4;
   size_t per_block = (j.n + want_blocks - 1) / want_blocks;
   per_block = ((per_block + step - 1) / step) * step;
@@ -364,9 +525,14 @@ template <class U> hipError_t run_radix(const RadixJob& j, hipStream_t s, Kernel
   const uint32_t tiles = sweep_tiles(j.n, P);
   const size_t zero_words = size_t(tiles) * 256 * (wide_status(j.n) ? 2 : 1);
   if (clk) clk->begin(s, "k_hist", double(j.n) * (j.records ? double(j.stride) + P : P));
-  hipLaunchKernelGGL((k_hist<U>), dim3(blocks ? blocks : 1), dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
-                     j.records, j.stride, j.key_offset, j.records ? static_cast<U*>(j.keys_a) : nullptr, j.n, j.mode,
-                     per_block, j.meta, static_cast<uint32_t*>(j.status), zero_words);
+  if (j.records)
+    hipLaunchKernelGGL((k_hist<U, true>), dim3(blocks ? blocks : 1), dim3(kHistThreads), 0, s, nullptr, j.records,
+                       j.stride, j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, per_block, j.meta,
+                       static_cast<uint32_t*>(j.status), zero_words);
+  else
+    hipLaunchKernelGGL((k_hist<U, false>), dim3(blocks ? blocks : 1), dim3(kHistThreads), 0, s,
+                       static_cast<const U*>(j.keys_a), nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, per_block,
+                       j.meta, static_cast<uint32_t*>(j.status), zero_words);
   if (clk) clk->end(s);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P);
   hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t>(j, s, clk) : run_sweeps<U, uint32_t>(j, s, clk);
