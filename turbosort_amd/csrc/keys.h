@@ -79,13 +79,77 @@ __host__ __device__ __forceinline__ int dtype_mode(int dt) {
   return kIdentity;
 }
 
-// Device bookkeeping of one sort: the all-digit histogram, the digit bases
-// (exclusive prefix of each digit's histogram), the tile tickets of each pass
-// and the list of live digits, i.e. digits on which the keys are not all
-// equal.  Passes are decided on the device, so a sort needs no host round trip.
+// Device bookkeeping of one sort, decided on the device so that a sort needs
+// no host round trip.
+//
+// Each scatter pass splits its input into kSegments contiguous segments and
+// runs one decoupled look-back chain per segment (tickets interleave the
+// segments, so every chain advances at a fraction of the tile rate).  The
+// first pass reads the input in its original order and uses position
+// segments; a later pass reads keys ordered by the previous digit and uses
+// value segments, the top three bits of the previous digit.  pair[p][s][x]
+// counts the keys of segment s (of digit p's pass) whose digit p is x; it is
+// filled in the single histogram read, and gives the digit bases, each
+// segment's offset inside each digit's run (segpre) and the segment bounds.
+constexpr int kSegments = 8;
+
+// Ticket order of a pass's tiles: round t holds tile t of every segment that
+// has more than t tiles, in segment order, and precedes round t+1.  So a
+// tile's predecessor in its segment always has a smaller ticket.  Maps
+// ticket k (< the total tile count) to (segment, round).
+__host__ __device__ __forceinline__ void ticket_to_tile(uint32_t k, const uint32_t (&nt)[kSegments], uint32_t* seg,
+                                                        uint32_t* round) {
+  uint32_t most = 0;
+  for (int s = 0; s < kSegments; ++s) most = nt[s] > most ? nt[s] : most;
+  // Largest t with F(t) <= k, F(t) = sum_s min(nt[s], t) (tiles in rounds < t).
+  uint32_t lo = 0, hi = most;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    uint32_t f = 0;
+    for (int s = 0; s < kSegments; ++s) f += nt[s] < mid ? nt[s] : mid;
+    if (f <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  uint32_t r = k;
+  for (int s = 0; s < kSegments; ++s) r -= nt[s] < lo ? nt[s] : lo;
+  uint32_t found = 0;
+  for (int s = 0; s < kSegments; ++s) {
+    if (nt[s] > lo) {
+      if (r == 0) {
+        found = uint32_t(s);
+        break;
+      }
+      --r;
+    }
+  }
+  *seg = found;
+  *round = lo;
+}
+
+// The geometry of one executed pass (the pass-th live digit), written by the
+// scan and read by that pass's scatter kernel in its first instructions.
+struct PassGeo {
+  uint32_t digit;
+  uint32_t total;  // tiles of the pass
+  uint32_t nt[kSegments];
+  uint32_t slot[kSegments];
+  uint32_t pad[2];
+  unsigned long long start[kSegments];
+  unsigned long long count[kSegments];
+};
+
 struct SortMeta {
-  unsigned long long hist[8][256];
+  PassGeo geo[8];
+  unsigned long long pair[8][kSegments][256];
   unsigned long long base[8][256];
+  unsigned long long segpre[8][kSegments][256];
+  unsigned long long seg_start[8][kSegments];
+  unsigned long long seg_count[8][kSegments];
+  uint32_t seg_tiles[8][kSegments];
+  uint32_t seg_slot[8][kSegments];
+  uint32_t total_tiles[8];
   uint32_t tickets[8];
   uint32_t n_live;
   uint32_t live[8];

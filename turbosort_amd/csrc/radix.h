@@ -31,32 +31,44 @@ struct KernelClock {
   }
 };
 
-// Tile geometry of the one-sweep scatter: 256 lanes (4 wave64s) per
-// workgroup, `ipt` keys per lane.
+// Tile geometry of the one-sweep scatter per key width: `threads` lanes per
+// workgroup (a multiple of 256; the first 256 lanes also carry one digit
+// each through the per-digit steps) and `ipt` keys per lane.  Measured on
+// MI355X at 2^28 keys: 4-byte keys are VALU-bound and prefer 12288-key tiles
+// at full occupancy; 8-byte keys are bound by the scatter's write runs and
+// prefer 8192-key tiles (256-byte average runs per digit).
+#ifndef TS_SWEEP_THREADS
+#define TS_SWEEP_THREADS 512
+#endif
 #ifndef TS_SWEEP_IPT_NARROW
-#define TS_SWEEP_IPT_NARROW 32
+#define TS_SWEEP_IPT_NARROW 24
 #endif
 #ifndef TS_SWEEP_IPT_WIDE
 #define TS_SWEEP_IPT_WIDE 16
 #endif
 // Predecessor tiles one look-back step reads at once.
 #ifndef TS_LOOKBACK_WINDOW
-#define TS_LOOKBACK_WINDOW 8
+#define TS_LOOKBACK_WINDOW 4
 #endif
-// Publish a tile's digit counts from an LDS histogram before ranking (1) or
-// from the ranking's own counts after it (0).
-#ifndef TS_EARLY_PUBLISH
-#define TS_EARLY_PUBLISH 1
-#endif
-constexpr int kSweepThreads = 256;
 template <int KB> struct SweepShape {
+  static constexpr int threads = TS_SWEEP_THREADS;
   static constexpr int ipt = KB >= 8 ? TS_SWEEP_IPT_WIDE : TS_SWEEP_IPT_NARROW;
-  static constexpr int tile = kSweepThreads * ipt;
+  static constexpr int tile = threads * ipt;
 };
 
+inline size_t sweep_tile(int key_bytes) {
+  return key_bytes >= 8 ? size_t(SweepShape<8>::tile) : size_t(SweepShape<4>::tile);
+}
+// Upper bound on the tiles of one pass: every segment may end in a partial tile.
 inline uint32_t sweep_tiles(size_t n, int key_bytes) {
-  const size_t tile = key_bytes >= 8 ? size_t(SweepShape<8>::tile) : size_t(SweepShape<4>::tile);
-  return uint32_t((n + tile - 1) / tile);
+  const size_t tile = sweep_tile(key_bytes);
+  return uint32_t((n + tile - 1) / tile + kSegments);
+}
+// Length of the first pass's position segments (a whole number of tiles).
+inline size_t first_segment(size_t n, int key_bytes) {
+  const size_t tile = sweep_tile(key_bytes);
+  const size_t per = (n + kSegments - 1) / kSegments;
+  return ((per + tile - 1) / tile) * tile;
 }
 
 // Look-back words are 32-bit (2 flag bits, 30 value bits) while every count

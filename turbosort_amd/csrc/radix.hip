@@ -32,14 +32,16 @@ template <class T> __device__ __forceinline__ T wave_inclusive_add(T v) {
   return v;
 }
 
-// Exclusive scan of one value per lane over a 256-lane workgroup.
+// Exclusive scan of one value per lane over the first 256 lanes of the
+// workgroup (every lane of the workgroup must call it; s_wave holds one slot
+// per wave).
 template <class T> __device__ __forceinline__ T block256_exclusive_add(T v, T* s_wave) {
   const unsigned tid = threadIdx.x, w = tid >> 6;
   T inc = wave_inclusive_add(v);
   if ((tid & 63) == 63) s_wave[w] = inc;
   __syncthreads();
   T off = 0;
-  for (unsigned i = 0; i < w; ++i) off += s_wave[i];
+  for (unsigned i = 0; i < w && i < 4; ++i) off += s_wave[i];
   __syncthreads();
   return off + inc - v;
 }
@@ -54,6 +56,20 @@ template <class U> __device__ __forceinline__ U first_lane(U x) {
   }
 }
 
+// Emits nothing; the compiler can no longer see x's value, so values it
+// derived from x earlier are recomputed rather than kept live.
+template <class U> __device__ __forceinline__ void opaque(U& x) {
+  if constexpr (sizeof(U) == 8) {
+    uint64_t t = uint64_t(x);
+    asm("" : "+v"(t));
+    x = U(t);
+  } else {
+    uint32_t t = uint32_t(x);
+    asm("" : "+v"(t));
+    x = U(t);
+  }
+}
+
 template <class U, int N> __device__ __forceinline__ void to_keys(U (&k)[N], int mode) {
   if (mode == kSigned) {
 #pragma unroll
@@ -65,8 +81,8 @@ template <class U, int N> __device__ __forceinline__ void to_keys(U (&k)[N], int
 }
 
 // ------------------------------------------------------------------ init
-__global__ void k_init(SortMeta* m) {
-  for (unsigned i = threadIdx.x; i < 8 * 256; i += blockDim.x) (&m->hist[0][0])[i] = 0;
+__global__ void k_init(SortMeta* m, int P) {
+  for (unsigned i = threadIdx.x; i < unsigned(P) * kSegments * 256; i += blockDim.x) (&m->pair[0][0][0])[i] = 0;
   if (threadIdx.x < 8) m->tickets[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     m->n_live = 0;
@@ -75,15 +91,19 @@ __global__ void k_init(SortMeta* m) {
 }
 
 // ------------------------------------------------------------- histogram
-// All P digit histograms in one read of the keys (histogram.rs:15-45).  Each
-// workgroup counts a contiguous chunk into LDS and adds its counts to the
-// global histogram once.  The forward key transform is fused into the load;
-// the by-key path (REC) reads keys out of records and writes the transformed
-// key array the scatter passes start from.
+// All digit histograms in one read of the keys (histogram.rs:15-45), split by
+// the segment each key falls in for that digit's pass: digit 0 by position
+// segment (first_segment() keys each), digit p > 0 by the top three bits of
+// digit p-1.  Each workgroup counts one contiguous chunk of one position
+// segment into LDS and adds its counts to the global table once.  The
+// forward key transform is fused into the load; the by-key path (REC) reads
+// keys out of records and writes the transformed key array the scatter
+// passes start from.
 //
-// A digit on which all of a wave's keys of one step agree (constant digits,
-// sorted runs) is counted with one LDS add instead of 64 colliding atomics.
-// The grid also clears the first look-back row.
+// A digit (with its segment bits) on which all of a wave's keys of one step
+// agree -- constant digits, sorted runs -- is counted with one LDS add
+// instead of 64 colliding atomics.  The grid also clears the first look-back
+// row.
 constexpr int kHistThreads = 256;
 template <class U> struct HistUnroll {
   static constexpr int value = sizeof(U) >= 8 ? 8 : 16;
@@ -92,20 +112,23 @@ template <class U> struct HistUnroll {
 template <class U, bool REC>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ keys, const unsigned char* __restrict__ rec,
                                                        size_t stride, size_t key_off, U* __restrict__ keys_out,
-                                                       size_t n, int mode, size_t per_block, SortMeta* m,
-                                                       uint32_t* zero_ptr, size_t zero_words) {
+                                                       size_t n, int mode, size_t seg0, uint32_t blocks_per_seg,
+                                                       size_t per_block, SortMeta* m, uint32_t* zero_ptr,
+                                                       size_t zero_words) {
   constexpr int P = int(sizeof(U));
   constexpr int UN = HistUnroll<U>::value;
-  __shared__ uint32_t h[P * 256];
+  __shared__ uint32_t h[P * kSegments * 256];
   const unsigned tid = threadIdx.x, lane = tid & 63;
-  for (unsigned i = tid; i < P * 256; i += kHistThreads) h[i] = 0;
+  for (unsigned i = tid; i < P * kSegments * 256; i += kHistThreads) h[i] = 0;
   {
     const size_t g = size_t(blockIdx.x) * kHistThreads + tid, gs = size_t(gridDim.x) * kHistThreads;
     for (size_t z = g; z < zero_words; z += gs) zero_ptr[z] = 0;
   }
   __syncthreads();
-  const size_t begin = size_t(blockIdx.x) * per_block;
-  const size_t end = begin + per_block < n ? begin + per_block : n;
+  const uint32_t s0 = blockIdx.x / blocks_per_seg;  // this block's position segment
+  const size_t seg_lo = min(n, size_t(s0) * seg0), seg_hi = min(n, seg_lo + seg0);
+  const size_t begin = min(seg_hi, seg_lo + size_t(blockIdx.x % blocks_per_seg) * per_block);
+  const size_t end = min(seg_hi, begin + per_block);
   for (size_t base = begin; base < end; base += size_t(kHistThreads) * UN) {
     const bool full = base + size_t(kHistThreads) * UN <= end;
     U k[UN];
@@ -125,13 +148,12 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ key
     }
     // Bits in which the wave's valid keys differ from its first key.
     const U k0 = first_lane(k[0]);
-    U diff = 0;
+    uint64_t diff = 0;
     uint32_t wave_count = 64u * UN;
 #pragma unroll
     for (int r = 0; r < UN; ++r) {
       const size_t i = base + size_t(r) * kHistThreads + tid;
-      const bool ok = full || i < end;
-      if (ok) diff |= U(k[r] ^ k0);
+      if (full || i < end) diff |= uint64_t(k[r] ^ k0);
     }
     if (!full) {
       wave_count = 0;
@@ -141,44 +163,105 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ key
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      const bool varies = unsigned((uint64_t(diff) >> (8 * p)) & 0xFF) != 0;
-      if (__ballot(varies) == 0) {
-        if (lane == 0) atomicAdd(&h[p * 256 + unsigned((uint64_t(k0) >> (8 * p)) & 0xFF)], wave_count);
+      // Digit p and, for p > 0, the segment bits (top three of digit p-1).
+      const uint64_t care = p == 0 ? 0xFFull : (0x7FFull << (8 * p - 3));
+      uint32_t* hp = h + p * kSegments * 256;
+      if (__ballot((diff & care) != 0) == 0) {
+        const uint32_t seg = p == 0 ? s0 : uint32_t(uint64_t(k0) >> (8 * p - 3)) & 7u;
+        if (lane == 0) atomicAdd(&hp[seg * 256 + (uint32_t(uint64_t(k0) >> (8 * p)) & 0xFFu)], wave_count);
       } else {
 #pragma unroll
         for (int r = 0; r < UN; ++r) {
           const size_t i = base + size_t(r) * kHistThreads + tid;
-          if (full || i < end) atomicAdd(&h[p * 256 + unsigned((uint64_t(k[r]) >> (8 * p)) & 0xFF)], 1u);
+          if (full || i < end) {
+            const uint32_t seg = p == 0 ? s0 : uint32_t(uint64_t(k[r]) >> (8 * p - 3)) & 7u;
+            atomicAdd(&hp[seg * 256 + (uint32_t(uint64_t(k[r]) >> (8 * p)) & 0xFFu)], 1u);
+          }
         }
       }
     }
   }
   __syncthreads();
-  for (unsigned i = tid; i < P * 256; i += kHistThreads) {
+  for (unsigned i = tid; i < P * kSegments * 256; i += kHistThreads) {
     const uint32_t c = h[i];
-    if (c) atomicAdd(&m->hist[0][0] + i, (unsigned long long)c);
+    if (c) atomicAdd(&m->pair[0][0][0] + i, (unsigned long long)c);
   }
 }
 
 // ------------------------------------------------------------------ scan
-// Digit bases (prefix_sum.rs:17-24) and live digits: a digit is dead when one
-// bin holds every key, as the reference decides from the first key's bin
-// (radix/mod.rs:229-239).
-__global__ __launch_bounds__(256) void k_scan(SortMeta* m, size_t n, int P) {
+// Per digit p: its histogram (the segments' counts summed), the digit bases
+// (prefix_sum.rs:17-24), each segment's offset inside each digit's run, the
+// segment bounds and tile counts of p's pass, and whether p is live.  A digit
+// is dead when one bin holds every key, as the reference decides from the
+// first key's bin (radix/mod.rs:229-239).  When digit p-1 is dead its value
+// segments collapse into one segment holding all keys in input order, which
+// is the plain single-chain pass, so the segments stay correct whichever
+// digits are live.
+__global__ __launch_bounds__(256) void k_scan(SortMeta* m, size_t n, int P, size_t seg0, uint32_t tile) {
   __shared__ unsigned long long s_wave[4];
+  __shared__ unsigned long long s_prev[256], s_prev_base[256];
+  __shared__ uint32_t s_tiles[kSegments];
   const unsigned tid = threadIdx.x;
   uint32_t n_live = 0;
   for (int p = 0; p < P; ++p) {
-    const unsigned long long c = m->hist[p][tid];
-    const unsigned long long e = block256_exclusive_add(c, s_wave);
+    unsigned long long tot = 0;
+#pragma unroll
+    for (int s = 0; s < kSegments; ++s) {
+      const unsigned long long c = m->pair[p][s][tid];
+      m->segpre[p][s][tid] = tot;
+      tot += c;
+    }
+    const unsigned long long e = block256_exclusive_add(tot, s_wave);
     m->base[p][tid] = e;
-    const int full = __syncthreads_or(c == (unsigned long long)n);
+    const int full = __syncthreads_or(tot == (unsigned long long)n);
     if (!full) {
       if (tid == 0) m->live[n_live] = uint32_t(p);
       ++n_live;
     }
+    if (tid < kSegments) {
+      unsigned long long st, ct;
+      if (p == 0) {
+        st = min((unsigned long long)n, (unsigned long long)tid * seg0);
+        ct = min((unsigned long long)n, st + seg0) - st;
+      } else {
+        ct = 0;
+        for (int x = 0; x < 32; ++x) ct += s_prev[tid * 32 + x];
+        st = s_prev_base[tid * 32];
+      }
+      m->seg_start[p][tid] = st;
+      m->seg_count[p][tid] = ct;
+      s_tiles[tid] = uint32_t((ct + tile - 1) / tile);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t slot = 0;
+      for (int s = 0; s < kSegments; ++s) {
+        m->seg_slot[p][s] = slot;
+        m->seg_tiles[p][s] = s_tiles[s];
+        slot += s_tiles[s];
+      }
+      m->total_tiles[p] = slot;
+    }
+    s_prev[tid] = tot;
+    s_prev_base[tid] = e;
   }
+  __syncthreads();
   if (tid == 0) m->n_live = n_live;
+  // Geometry of each executed pass, by execution order.
+  for (uint32_t i = 0; i < n_live; ++i) {
+    const uint32_t p = m->live[i];
+    PassGeo* g = &m->geo[i];
+    if (tid == 0) {
+      g->digit = p;
+      g->total = m->total_tiles[p];
+    }
+    if (tid < kSegments) {
+      g->nt[tid] = m->seg_tiles[p][tid];
+      g->slot[tid] = m->seg_slot[p][tid];
+      g->start[tid] = m->seg_start[p][tid];
+      g->count[tid] = m->seg_count[p][tid];
+    }
+  }
 }
 
 // ------------------------------------------------------------- one sweep
@@ -200,6 +283,7 @@ template <class U> struct SweepArgs {
   int mode_in;   // forward transform applied by the first pass
   int mode_out;  // inverse transform applied by the last pass
   const SortMeta* meta;
+  const PassGeo* geo;  // meta->geo, read-only in the scatter
   uint32_t* tickets;
   uint32_t* error;  // set when a look-back wait exceeds its bound
   uint32_t num_tiles;
@@ -231,7 +315,7 @@ __device__ __forceinline__ uint32_t keep_equal(uint32_t x, uint32_t vote, uint32
 // equal digits are found with eight ballots: lane l keeps the lanes whose
 // vote on every digit bit equals its own, i.e. ~(vote ^ mybit) per bit.
 template <class U, int IPT, bool FULL>
-__device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[IPT], unsigned shift,
+__device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[(IPT + 1) / 2], unsigned shift,
                                              uint32_t* whist, uint32_t lb, uint32_t valid) {
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
@@ -268,84 +352,76 @@ __device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[I
     // its group holds only the valid lanes sharing that digit, so it writes
     // their count (or the unchanged count when there are none).
     whist[d] = before + uint32_t(__popc(lo) + __popc(hi));
-    rk[j] = before + below;
+    // Ranks are below 64 * IPT, so two share a register.
+    if (j & 1)
+      rk[j >> 1] |= (before + below) << 16;
+    else
+      rk[j >> 1] = before + below;
     (void)ok;
   }
 }
 
 // One stable scatter pass by the byte digit meta->live[pass] (scatter.rs:26-40).
 //
-// Tiles are claimed in launch order through a ticket, so every tile a tile
-// waits on has already started.  Each wave ranks its IPT x 64 keys stably
-// (rank_in_wave); a cross-wave scan gives tile-local positions.  The tile's
-// digit counts are published, the exclusive prefix over earlier tiles is
-// gathered by decoupled look-back, and the tile is staged in LDS in digit
-// order so that each digit's keys leave as one coalesced run.
+// Tiles are claimed through a ticket in an order that interleaves the pass's
+// segments (round t of every segment before round t+1), so every tile a tile
+// waits on has already started, and each segment's look-back chain advances
+// at 1/kSegments of the tile rate.  Each wave ranks its IPT x 64 keys stably
+// (rank_in_wave); the tile's digit counts are published first from an LDS
+// histogram; the exclusive prefix over the segment's earlier tiles is
+// gathered by decoupled look-back, several predecessors per step; and the
+// tile is staged in LDS in digit order so that each digit's keys leave as one
+// coalesced run.  A key's position is digit base + the count of its digit in
+// earlier segments + earlier tiles of its segment + its rank in the tile.
 //
 // Pass 0 of a by-key sort generates the values (positions 0..n-1); the last
 // pass applies the inverse key transform, and in a by-key sort writes only the
 // order to final_vals.
-template <class U, bool VALS, class S>
-__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __restrict__ status, int pass) {
+// LDS of one scatter tile, declared once in the kernel.
+template <class U, bool VALS, int IPT> struct SweepLds {
+  static constexpr int NW = SweepShape<sizeof(U)>::threads / 64;
+  static constexpr int TILE = SweepShape<sizeof(U)>::threads * IPT;
+  uint32_t whist[NW][256];  // per-wave digit counts, then staging bases
+  U keys[TILE];
+  uint32_t vals[VALS ? TILE : 1];
+  unsigned long long gofs[256];
+  uint32_t wave[NW];
+  uint32_t tcnt[256];
+  uint32_t ticket;
+};
+
+// Everything a tile does after it knows where it is: load, publish its digit
+// counts, rank, look back, stage, store.  FULL tiles (all but each segment's
+// last) run without any per-key bounds test.
+template <class U, bool VALS, class S, bool FULL>
+__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>::ipt>& L, const SweepArgs<U>& a,
+                                           S* __restrict__ status, int pass, unsigned shift, bool first, bool last,
+                                           size_t tile_base, uint32_t valid, size_t first_slot, size_t slot,
+                                           bool first_round, unsigned long long digit_base) {
   constexpr int IPT = SweepShape<sizeof(U)>::ipt;
-  constexpr int NW = kSweepThreads / 64;
-  constexpr int TILE = kSweepThreads * IPT;
+  constexpr int THREADS = SweepShape<sizeof(U)>::threads;
+  constexpr int NW = THREADS / 64;
+  constexpr int TILE = THREADS * IPT;
   constexpr int VB = Flags<S>::kBits;
   constexpr S kAgg = S(1) << VB, kPre = S(2) << VB, kMask = kAgg - 1;
-
-  __shared__ uint32_t s_whist[NW][256];
-  __shared__ U s_keys[TILE];
-  __shared__ uint32_t s_vals[VALS ? TILE : 1];
-  __shared__ unsigned long long s_gofs[256];
-  __shared__ uint32_t s_wave[NW];
-  __shared__ uint32_t s_tile;
-#if TS_EARLY_PUBLISH
-  __shared__ uint32_t s_tcnt[256];
-#endif
-
-  const uint32_t n_live = a.meta->n_live;
-  if (uint32_t(pass) >= n_live) return;
-  const unsigned digit = a.meta->live[pass];
-  const unsigned shift = 8 * digit;
-  const bool first = pass == 0;
-  const bool last = uint32_t(pass) + 1 == n_live;
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t lb = w * (64 * IPT) + lane;  // this lane's first key in the tile
   const U* __restrict__ src = (pass & 1) ? a.keys_b : a.keys_a;
   U* __restrict__ dst = (pass & 1) ? a.keys_a : a.keys_b;
   const uint32_t* __restrict__ vsrc = (pass & 1) ? a.vals_b : a.vals_a;
   uint32_t* __restrict__ vdst = (last && a.final_vals) ? a.final_vals : ((pass & 1) ? a.vals_a : a.vals_b);
-  const bool write_keys = !(VALS && last && a.final_vals);
-
-  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (tid == 0) s_tile = atomicAdd(&a.tickets[pass], 1u);
-  for (unsigned i = tid; i < NW * 256; i += kSweepThreads) (&s_whist[0][0])[i] = 0;
-#if TS_EARLY_PUBLISH
-  s_tcnt[tid] = 0;
-#endif
-  __syncthreads();
-  const uint32_t tile = s_tile;
   S* row = status + size_t(pass & 1) * a.num_tiles * 256;
-  S* next_row = status + size_t((pass + 1) & 1) * a.num_tiles * 256;
-  next_row[size_t(tile) * 256 + tid] = 0;  // the next pass's look-back row
-
-  const size_t n = a.n;
-  const size_t tile_base = size_t(tile) * TILE;
-  const uint32_t valid = n - tile_base >= size_t(TILE) ? uint32_t(TILE) : uint32_t(n - tile_base);
-  const bool full = valid == uint32_t(TILE);
-  const uint32_t lb = w * (64 * IPT) + lane;  // this lane's first key in the tile
+  const bool dl = tid < 256;  // this lane also carries digit tid
+  auto in_tile = [&](int j) { return FULL || lb + uint32_t(j) * 64 < valid; };
 
   U k[IPT];
   uint32_t v[VALS ? IPT : 1];
   {
     const U* __restrict__ ts = src + tile_base;
-    if (full) {
 #pragma unroll
-      for (int j = 0; j < IPT; ++j) k[j] = ts[lb + uint32_t(j) * 64];
-    } else {
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const uint32_t li = lb + uint32_t(j) * 64;
-        k[j] = ts[li < valid ? li : valid - 1];
-      }
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t li = lb + uint32_t(j) * 64;
+      k[j] = ts[FULL || li < valid ? li : valid - 1];
     }
     if (VALS) {
       if (first) {
@@ -356,29 +432,30 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
           const uint32_t li = lb + uint32_t(j) * 64;
-          v[j] = tv[full || li < valid ? li : valid - 1];
+          v[j] = tv[FULL || li < valid ? li : valid - 1];
         }
       }
     }
     if (first) to_keys(k, a.mode_in);
   }
 
-  S* my = row + size_t(tile) * 256 + tid;
-#if TS_EARLY_PUBLISH
   // Publish the tile's digit counts before ranking, so that later tiles
   // rarely find this one unpublished.
+  S* my = row + slot * 256 + tid;
 #pragma unroll
   for (int j = 0; j < IPT; ++j)
-    if (full || lb + uint32_t(j) * 64 < valid) atomicAdd(&s_tcnt[digit_of(k[j], shift)], 1u);
+    if (in_tile(j)) atomicAdd(&L.tcnt[digit_of(k[j], shift)], 1u);
   __syncthreads();
-  __hip_atomic_store(my, S(s_tcnt[tid]) | (tile == 0 ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#endif
+  if (dl) __hip_atomic_store(my, S(L.tcnt[tid]) | (first_round ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // Emit nothing, but make the keys opaque so that the digits computed above
+  // are recomputed (one v_bfe each) instead of being held in IPT registers.
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) opaque(k[j]);
 
-  uint32_t rk[IPT];
-  if (full)
-    rank_in_wave<U, IPT, true>(k, rk, shift, s_whist[w], lb, valid);
-  else
-    rank_in_wave<U, IPT, false>(k, rk, shift, s_whist[w], lb, valid);
+  uint32_t rk[(IPT + 1) / 2];
+  rank_in_wave<U, IPT, FULL>(k, rk, shift, L.whist[w], lb, valid);
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) opaque(k[j]);  // again: recompute digits when staging
   __syncthreads();
 
   // Per digit: exclusive offsets of the waves, and the tile's count.
@@ -387,33 +464,49 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
 #pragma unroll
   for (int ww = 0; ww < NW; ++ww) {
     wexcl[ww] = cnt;
-    cnt += s_whist[ww][tid];
+    if (dl) cnt += L.whist[ww][tid];
   }
-#if !TS_EARLY_PUBLISH
-  __hip_atomic_store(my, S(cnt) | (tile == 0 ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#endif
-  const uint32_t excl_in_tile = block256_exclusive_add(cnt, s_wave);
+  const uint32_t excl_in_tile = block256_exclusive_add(cnt, L.wave);
+  if (dl) {
 #pragma unroll
-  for (int ww = 0; ww < NW; ++ww) s_whist[ww][tid] = excl_in_tile + wexcl[ww];
+    for (int ww = 0; ww < NW; ++ww) L.whist[ww][tid] = excl_in_tile + wexcl[ww];
+  }
+  __syncthreads();
 
-  // Decoupled look-back over earlier tiles for this lane's digit.  Every tile
-  // with a smaller ticket is running and publishes without waiting on later
-  // tiles, so the walk ends; the spin bound only turns a broken invariant
-  // into a flagged wrong result instead of a hung GPU.
-  // Each step reads a window of kWindow predecessors at once and consumes
-  // them newest first until one holds an inclusive prefix.
+  // Stage the tile in digit order.  It does not depend on the look-back, so it
+  // comes first and the keys and ranks die before the wait.
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    if (in_tile(j)) {
+      const uint32_t d = digit_of(k[j], shift);
+      const uint32_t pos = L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
+      L.keys[pos] = k[j];
+      if (VALS) L.vals[pos] = v[j];
+    }
+    // Keep the scheduler from hoisting every iteration's LDS read ahead of
+    // the writes, which costs more registers than it hides latency.
+    if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+  }
+
+  // Decoupled look-back over the segment's earlier tiles for this lane's
+  // digit, reading a window of predecessors per step and consuming them
+  // newest first until one holds an inclusive prefix.  Every tile with a
+  // smaller ticket is running and publishes without waiting on later tiles,
+  // so the walk ends; the spin bound only turns a broken invariant into a
+  // flagged wrong result instead of a hung GPU.
   unsigned long long before_tiles = 0;
-  if (tile > 0) {
+  if (dl && !first_round) {
     constexpr int kWindow = TS_LOOKBACK_WINDOW;
-    int64_t j = int64_t(tile) - 1;
+    int64_t j = int64_t(slot) - 1;
+    const int64_t stop = int64_t(first_slot);
     uint32_t spins = 0;
     while (true) {
       S st[kWindow];
 #pragma unroll
       for (int q = 0; q < kWindow; ++q)
-        st[q] = j - q >= 0 ? __hip_atomic_load(row + size_t(j - q) * 256 + tid, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT)
-                           : S(0);
+        st[q] = j - q >= stop ? __hip_atomic_load(row + size_t(j - q) * 256 + tid, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : S(0);
       int used = 0;
       bool done = false;
 #pragma unroll
@@ -422,7 +515,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
         if (fl == 0) break;  // not published yet: wait for it
         before_tiles += (unsigned long long)(st[q] & kMask);
         ++used;
-        if (fl == 2 || j - q == 0) {
+        if (fl == 2 || j - q == stop) {
           done = true;
           break;
         }
@@ -439,34 +532,105 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<U> a, S* __re
     }
     __hip_atomic_store(my, S(before_tiles + cnt) | kPre, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  s_gofs[tid] = a.meta->base[digit][tid] + before_tiles - excl_in_tile;
+  if (dl) L.gofs[tid] = digit_base + before_tiles - excl_in_tile;
   __syncthreads();
 
-  // Stage the tile in digit order.
+  // Write each digit's run of the tile contiguously.  The output transform
+  // and whether keys are written are uniform, so each case is its own loop.
+  auto store = [&](auto transform, bool keys_out) {
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    if (full || lb + uint32_t(j) * 64 < valid) {
-      const uint32_t d = digit_of(k[j], shift);
-      const uint32_t pos = s_whist[w][d] + rk[j];
-      s_keys[pos] = k[j];
-      if (VALS) s_vals[pos] = v[j];
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t s = uint32_t(j) * THREADS + tid;
+      if (FULL || s < valid) {
+        const U kk = L.keys[s];
+        const size_t o = size_t(L.gofs[digit_of(kk, shift)]) + s;
+        if (keys_out) dst[o] = transform(kk);
+        if (VALS) vdst[o] = L.vals[s];
+      }
+      if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
-  }
-  __syncthreads();
-
-  // Write each digit's run of the tile contiguously.
+  };
+  const bool write_keys = !(VALS && last && a.final_vals);
   const int out_mode = last ? a.mode_out : int(kIdentity);
+  if (!write_keys)
+    store([](U x) { return x; }, false);
+  else if (out_mode == kSigned)
+    store([](U x) { return from_key(x, kSigned); }, true);
+  else if (out_mode == kFloat)
+    store([](U x) { return from_key(x, kFloat); }, true);
+  else
+    store([](U x) { return x; }, true);
+  (void)TILE;
+}
+
+// One stable scatter pass by the byte digit geo[pass].digit (scatter.rs:26-40).
+//
+// Tiles are claimed through a ticket in an order that interleaves the pass's
+// segments (round t of every segment before round t+1), so every tile a tile
+// waits on has already started, and each segment's look-back chain advances
+// at 1/kSegments of the tile rate.  Each wave ranks its IPT x 64 keys stably
+// (rank_in_wave); the tile's digit counts are published first from an LDS
+// histogram; the exclusive prefix over the segment's earlier tiles is
+// gathered by decoupled look-back, several predecessors per step; and the
+// tile is staged in LDS in digit order so that each digit's keys leave as one
+// coalesced run.  A key's position is digit base + the count of its digit in
+// earlier segments + earlier tiles of its segment + its rank in the tile.
+//
+// Pass 0 of a by-key sort generates the values (positions 0..n-1); the last
+// pass applies the inverse key transform, and in a by-key sort writes only the
+// order to final_vals.
+template <class U, bool VALS, class S>
+__global__ __launch_bounds__(SweepShape<sizeof(U)>::threads) void k_sweep(SweepArgs<U> a, S* __restrict__ status, int pass) {
+  constexpr int IPT = SweepShape<sizeof(U)>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U)>::threads;
+  constexpr int NW = THREADS / 64;
+  constexpr int TILE = THREADS * IPT;
+  __shared__ SweepLds<U, VALS, IPT> L;
+
+  const SortMeta* __restrict__ meta = a.meta;
+  const uint32_t n_live = meta->n_live;
+  if (uint32_t(pass) >= n_live) return;
+  const unsigned tid = threadIdx.x;
+  if (tid == 0) L.ticket = atomicAdd(&a.tickets[pass], 1u);
+  // The pass's geometry, from wave-uniform addresses (scalar loads) issued
+  // while the ticket is in flight.
+  const PassGeo* __restrict__ g = a.geo + pass;
+  const unsigned digit = g->digit;
+  const uint32_t total = g->total;
+  uint32_t nt[kSegments];
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint32_t s = uint32_t(j) * kSweepThreads + tid;
-    if (full || s < valid) {
-      const U kk = s_keys[s];
-      const uint32_t d = digit_of(kk, shift);
-      const size_t o = size_t(s_gofs[d]) + s;
-      if (write_keys) dst[o] = out_mode == kIdentity ? kk : from_key(kk, out_mode);
-      if (VALS) vdst[o] = s_vals[s];
-    }
-  }
+  for (int s = 0; s < kSegments; ++s) nt[s] = g->nt[s];
+  const unsigned shift = 8 * digit;
+  const bool first = pass == 0;
+  const bool last = uint32_t(pass) + 1 == n_live;
+
+  for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
+  if (tid < 256) L.tcnt[tid] = 0;
+  __syncthreads();
+  const uint32_t ticket = __builtin_amdgcn_readfirstlane(L.ticket);
+  S* next_row = status + size_t((pass + 1) & 1) * a.num_tiles * 256;
+  if (tid < 256) next_row[size_t(ticket) * 256 + tid] = 0;  // the next pass's look-back row
+  if (ticket >= total) return;
+  uint32_t seg, round;
+  ticket_to_tile(ticket, nt, &seg, &round);  // uniform: stays in scalar registers
+  seg = __builtin_amdgcn_readfirstlane(seg);
+  round = __builtin_amdgcn_readfirstlane(round);
+
+  const size_t seg_lo = g->start[seg];
+  const size_t seg_hi = seg_lo + g->count[seg];
+  const size_t tile_base = seg_lo + size_t(round) * TILE;
+  const uint32_t valid = seg_hi - tile_base >= size_t(TILE) ? uint32_t(TILE) : uint32_t(seg_hi - tile_base);
+  const size_t first_slot = g->slot[seg];
+  const size_t slot = first_slot + round;
+  // This lane's digit: its base and its count in earlier segments.
+  const unsigned long long digit_base =
+      tid < 256 ? meta->base[digit][tid] + meta->segpre[digit][seg][tid] : 0ull;
+  if (valid == uint32_t(TILE))
+    run_tile<U, VALS, S, true>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot, slot,
+                                 round == 0, digit_base);
+  else
+    run_tile<U, VALS, S, false>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot, slot,
+                                  round == 0, digit_base);
 }
 
 // An odd number of live passes leaves the keys in the scratch half.
@@ -496,6 +660,7 @@ template <class U, class S> hipError_t run_sweeps(const RadixJob& j, hipStream_t
   a.mode_in = j.records ? int(kIdentity) : j.mode;
   a.mode_out = a.mode_in;
   a.meta = j.meta;
+  a.geo = j.meta->geo;
   a.tickets = j.meta->tickets;
   a.error = &j.meta->error;
   a.num_tiles = tiles;
@@ -504,9 +669,9 @@ template <class U, class S> hipError_t run_sweeps(const RadixJob& j, hipStream_t
   for (int p = 0; p < int(sizeof(U)); ++p) {
     if (clk) clk->begin(s, "k_sweep", bytes);
     if (vals)
-      hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(kSweepThreads), 0, s, a, static_cast<S*>(j.status), p);
+      hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
     else
-      hipLaunchKernelGGL((k_sweep<U, false, S>), dim3(tiles), dim3(kSweepThreads), 0, s, a, static_cast<S*>(j.status), p);
+      hipLaunchKernelGGL((k_sweep<U, false, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
     if (clk) clk->end(s);
   }
   return hipGetLastError();
@@ -514,27 +679,28 @@ template <class U, class S> hipError_t run_sweeps(const RadixJob& j, hipStream_t
 
 template <class U> hipError_t run_radix(const RadixJob& j, hipStream_t s, KernelClock* clk) {
   constexpr int P = int(sizeof(U));
-  hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, s, j.meta);
-  // Histogram: a few workgroups per CU, each over a contiguous chunk.
+  hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, s, j.meta, P);
+  // Histogram: blocks_per_seg workgroups per position segment, each over a
+  // contiguous chunk of it.
+  const size_t seg0 = first_segment(j.n, P);
+  const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * 4 / kSegments));
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
-  const size_t want_blocks = size_t(j.num_cus) * 4;
-  size_t per_block = (j.n + want_blocks - 1) / want_blocks;
-  per_block = ((per_block + step - 1) / step) * step;
-  if (per_block == 0) per_block = step;
-  const size_t blocks = (j.n + per_block - 1) / per_block;
+  size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
+  per_block = std::max(step, ((per_block + step - 1) / step) * step);
   const uint32_t tiles = sweep_tiles(j.n, P);
   const size_t zero_words = size_t(tiles) * 256 * (wide_status(j.n) ? 2 : 1);
+  const dim3 hgrid(blocks_per_seg * kSegments);
   if (clk) clk->begin(s, "k_hist", double(j.n) * (j.records ? double(j.stride) + P : P));
   if (j.records)
-    hipLaunchKernelGGL((k_hist<U, true>), dim3(blocks ? blocks : 1), dim3(kHistThreads), 0, s, nullptr, j.records,
-                       j.stride, j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, per_block, j.meta,
+    hipLaunchKernelGGL((k_hist<U, true>), hgrid, dim3(kHistThreads), 0, s, nullptr, j.records, j.stride,
+                       j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
                        static_cast<uint32_t*>(j.status), zero_words);
   else
-    hipLaunchKernelGGL((k_hist<U, false>), dim3(blocks ? blocks : 1), dim3(kHistThreads), 0, s,
-                       static_cast<const U*>(j.keys_a), nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, per_block,
-                       j.meta, static_cast<uint32_t*>(j.status), zero_words);
+    hipLaunchKernelGGL((k_hist<U, false>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a), nullptr,
+                       size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
+                       static_cast<uint32_t*>(j.status), zero_words);
   if (clk) clk->end(s);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(P)));
   hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t>(j, s, clk) : run_sweeps<U, uint32_t>(j, s, clk);
   if (e != hipSuccess) return e;
   const unsigned cb = unsigned(std::min<size_t>((j.n + 255) / 256, size_t(j.num_cus) * 8));
