@@ -87,9 +87,9 @@ def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx):
     ctx.set_profiling(True)
     kt = []
     for i in range(3):
-        bufs[i].copy_(src)
+        bufs[0].copy_(src)
         torch.cuda.synchronize()
-        ts.sort(bufs[i], ctx=ctx)
+        ts.sort(bufs[0], ctx=ctx)
         torch.cuda.synchronize()
         kt.append(ctx.kernel_times())
     ctx.set_profiling(False)
@@ -112,6 +112,25 @@ def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx):
         sweep_gbs=sweep_bytes / (sweep_ms * 1e-3) / 1e9, hist_gbs=w * n / (hist_ms * 1e-3) / 1e9,
         algo_bytes_per_key=w * (1 + 2 * live), total_gbs=algo_bytes / (wall / steps) / 1e9,
     )
+
+
+def pmc_traffic(workload: str, log2n: int):
+    """HBM bytes per k_sweep launch from the committed rocprofv3 counter runs.
+
+    ``profiles/k_sweep_<workload>_2p<log2n>.md`` holds FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by what the same counters read for a copy
+    of known size, summed into a ``traffic_bytes_per_launch:`` line.  None when
+    no such summary exists for this workload and size.
+    """
+    path = os.path.join(HERE, "profiles", f"k_sweep_{workload}_2p{log2n}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(budget_s: float):
@@ -182,7 +201,7 @@ def main():
         "config": {"workload": head["desc"], "n": head["n"], "key_bytes": head["w"],
                    "algorithmic_bytes_per_key": head["algo_bytes_per_key"], "device_resident": True},
         "roofline": {"bound": "hbm", "achieved": sweep_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": pmc_traffic(a.workload, a.log2n),
                      "kernel": "k_sweep (one-sweep scatter pass; 2*w bytes per key per launch)",
                      "kernel_ms": head["sweep_ms"]},
         "whole_sort": {"algorithmic_GBps": head["total_gbs"], "frac": head["total_gbs"] / HBM_PEAK_GBS,

@@ -114,9 +114,11 @@ class Context:
             self._lib.ts_ctx_destroy(self._h)
             self._h = ctypes.c_void_p()
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # No HIP calls while the interpreter (and the HIP runtime) shuts down.
-        if sys.is_finalizing():
+        # The module global ``sys`` may already be gone then, hence the
+        # default argument bound at class creation.
+        if _finalizing():
             return
         try:
             self.close()
