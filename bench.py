@@ -38,7 +38,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="u32", choices=sorted(WORKLOADS))
     p.add_argument("--log2n", type=int, default=28)
-    p.add_argument("--also", default="u64,u64z", help="extra workloads reported beside the headline")
+    p.add_argument("--also", default="u64,u64z,segmented,by_key",
+                   help="extra workloads reported beside the headline (u32, u64, u64z, segmented, by_key)")
     p.add_argument("--cpu-seconds", type=float, default=8.0, help="wall budget of the CPU baseline")
     p.add_argument("--no-cpu-baseline", action="store_true")
     return p.parse_args()
@@ -114,6 +115,161 @@ def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx):
     )
 
 
+# ------------------------------------------------------------ generators
+# SplitMix64 (Vigna's splitmix64.c): output i of the stream with state s is
+# mix(s + (i + 1) * 0x9E3779B97F4A7C15); the device fill kernel computes the same.
+def np_splitmix64(seed: int, n: int):
+    import numpy as np
+
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def np_unit(bits):
+    import numpy as np
+
+    return (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def segment_offsets(seed: int, n_segments: int):
+    """Offsets of n_segments log-uniform lengths clamp(floor(exp(U(ln 2, ln 16385))), 2, 16384)."""
+    import numpy as np
+
+    u = np_unit(np_splitmix64(seed, n_segments))
+    a, b = np.log(2.0), np.log(16385.0)
+    lens = np.clip(np.floor(np.exp(a + u * (b - a))), 2, 16384).astype(np.int64)
+    off = np.zeros(n_segments + 1, dtype=np.int64)
+    off[1:] = np.cumsum(lens)
+    return off
+
+
+def score_records(n: int, seed: int):
+    """n 16-byte records {u32 id = i, f32 score, u64 payload}.
+
+    score = f32 of a standard normal (Box-Muller on two SplitMix64 streams,
+    states seed and seed + 2^32); 1 % of the records (stream seed + 2^33
+    below 0.01) get one of {+NaN, -NaN, +inf, -inf, -0.0} in equal shares;
+    then 20 % (stream seed + 3*2^32 below 0.2) copy the score of the record
+    whose index is stream seed + 2^34 modulo n; payload = stream seed + 5*2^32.
+    """
+    import numpy as np
+
+    s = lambda k: seed + k * (1 << 32)  # noqa: E731
+    u1 = np_unit(np_splitmix64(s(0), n))
+    u2 = np_unit(np_splitmix64(s(1), n))
+    score = (np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)).astype(np.float32)
+    del u1, u2
+    bits = score.view(np.uint32)
+    r = np_unit(np_splitmix64(s(2), n))
+    sp = r < 0.01
+    specials = np.array([0x7FC00000, 0xFFC00000, 0x7F800000, 0xFF800000, 0x80000000], dtype=np.uint32)
+    bits[sp] = specials[np.minimum((r[sp] * 500).astype(np.int64), 4)]
+    dup = np_unit(np_splitmix64(s(3), n)) < 0.2
+    src = (np_splitmix64(s(4), n) % np.uint64(n)).astype(np.int64)
+    bits[dup] = bits[src[dup]]
+    rec = np.zeros(n, dtype=np.dtype([("id", "<u4"), ("score", "<u4"), ("payload", "<u8")]))
+    rec["id"] = np.arange(n, dtype=np.uint32)
+    rec["score"] = bits
+    rec["payload"] = np_splitmix64(s(5), n)
+    return rec
+
+
+def timed(fn, steps: int):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        fn(i)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7):
+    """Batched segmented key-value sort: 10^5 log-uniform segments in [2, 16384],
+    u32 keys uniform (SplitMix64 state seed + 2^32), u32 values = global index."""
+    import torch
+
+    import turbosort_amd as ts
+
+    dev = torch.device("cuda", 0)
+    off = torch.from_numpy(segment_offsets(seed, n_segments)).to(dev)
+    n = int(off[-1])
+    src = torch.empty(n, dtype=torch.uint32, device=dev)
+    ts.fill_splitmix64(src, seed + (1 << 32), ctx=ctx)
+    warm = 1
+    ks = [src.clone() for _ in range(steps + warm)]
+    vs = [torch.arange(n, dtype=torch.int32, device=dev) for _ in range(steps + warm)]
+    run = lambda i: ts.segmented_sort(ks[i], off, vs[i], max_segment_len=16384, ctx=ctx)  # noqa: E731
+    for i in range(warm):
+        run(i)
+    dt = timed(lambda i: run(warm + i), steps)
+    # Correct iff values stay in their segments as a permutation, each key is
+    # the input key at its value, keys do not decrease and ties keep order.
+    k, v = ks[-1].view(torch.int32), vs[-1].long()
+    seg = torch.repeat_interleave(torch.arange(n_segments, device=dev), off[1:] - off[:-1])
+    seen = torch.zeros(n, dtype=torch.bool, device=dev)
+    seen[v] = True
+    kk = k.long() & 0xFFFFFFFF
+    same = seg[1:] == seg[:-1]
+    ok = (bool(torch.equal(seg[v], seg)) and bool(seen.all()) and bool(torch.equal(src.view(torch.int32)[v], k))
+          and not bool((same & (kk[1:] < kk[:-1])).any())
+          and not bool((same & (kk[1:] == kk[:-1]) & (v[1:] <= v[:-1])).any()))
+    del ks, vs, seg, seen, kk, same
+    torch.cuda.empty_cache()
+    return dict(desc="batched segmented key-value sort, 10^5 log-uniform segments in [2, 16384] (SplitMix64 "
+                     "state 7), u32 keys uniform, u32 values = global index", n=n, segments=n_segments,
+                ms_per_step=dt * 1e3, pairs_per_s=n / dt, algo_bytes_per_pair=16,
+                algorithmic_GBps=16 * n / dt / 1e9, roofline_frac=16 * n / dt / 1e9 / HBM_PEAK_GBS, ok=ok)
+
+
+def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5):
+    """Stable sort_by_key of 10^8 16-byte records keyed on an f32 score at
+    offset 4, then apply_order on a second u64 slice (SplitMix64 state 6)."""
+    import numpy as np
+    import torch
+
+    import turbosort_amd as ts
+
+    dev = torch.device("cuda", 0)
+    rec = score_records(n, seed)
+    src = torch.from_numpy(rec.view(np.uint8).reshape(n, 16)).to(dev)
+    del rec
+    other_src = torch.empty(n, dtype=torch.uint64, device=dev)
+    ts.fill_splitmix64(other_src, 6, ctx=ctx)
+    buf = ts.SortByKeyBuffer.with_capacity(n, device=dev, key_dtype=torch.float32, record_bytes=16)
+    key = ts.KeyField(4, torch.float32)
+    warm = 1
+    rs = [src.clone() for _ in range(steps + warm)]
+    for i in range(warm):
+        ts.sort_by_key_with_buffer(rs[i], buf, key)
+    dt = timed(lambda i: ts.sort_by_key_with_buffer(rs[warm + i], buf, key), steps)
+    others = [other_src.clone() for _ in range(steps)]
+    dt_apply = timed(lambda i: buf.apply_order(others[i]), steps)
+    # order() is the stable argsort iff it is a permutation along which the
+    # scores' radix keys do not decrease and equal keys keep increasing order.
+    order = buf.order().view(torch.int32).long()
+    sbits = src[:, 4:8].contiguous().view(torch.int32).reshape(-1).long() & 0xFFFFFFFF
+    radix = torch.where(sbits >= 0x80000000, sbits ^ 0xFFFFFFFF, sbits | 0x80000000)
+    rk = radix[order]
+    seen = torch.zeros(n, dtype=torch.bool, device=dev)
+    seen[order] = True
+    ok = (bool(seen.all()) and not bool((rk[1:] < rk[:-1]).any())
+          and not bool(((rk[1:] == rk[:-1]) & (order[1:] <= order[:-1])).any())
+          and bool(torch.equal(rs[-1], src[order])) and bool(torch.equal(others[-1].view(torch.int64),
+                                                                       other_src.view(torch.int64)[order])))
+    del rs, others, order, sbits, radix, rk, seen, src
+    torch.cuda.empty_cache()
+    return dict(desc="stable sort_by_key, 10^8 16-byte records {u32 id, f32 score, u64 payload} keyed on the "
+                     "score (N(0,1), 1 % NaN/inf/-0.0, 20 % duplicated), records gathered; apply_order on u64[10^8]",
+                n=n, ms_per_step=dt * 1e3, records_per_s=n / dt, algo_bytes_per_record=120,
+                algorithmic_GBps=120 * n / dt / 1e9, roofline_frac=120 * n / dt / 1e9 / HBM_PEAK_GBS,
+                apply_order_ms=dt_apply * 1e3, apply_order_GBps=20 * n / dt_apply / 1e9, ok=ok)
+
+
 def pmc_traffic(workload: str, log2n: int):
     """HBM bytes per k_sweep launch from the committed rocprofv3 counter runs.
 
@@ -179,9 +335,14 @@ def main():
     head = run_workload(a.workload, a.log2n, a.steps, a.warmup, ctx)
     also = {}
     for name in [x for x in a.also.split(",") if x and x != a.workload]:
-        r = run_workload(name, a.log2n, max(3, a.steps // 3), 1, ctx)
-        also[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items() if k != "desc"}
-        also[name]["roofline_frac"] = round(r["total_gbs"] / HBM_PEAK_GBS, 4)
+        if name == "segmented":
+            r = run_segmented(max(3, a.steps // 2), ctx)
+        elif name == "by_key":
+            r = run_by_key(max(3, a.steps // 3), ctx)
+        else:
+            r = run_workload(name, a.log2n, max(3, a.steps // 3), 1, ctx)
+            r["roofline_frac"] = r["total_gbs"] / HBM_PEAK_GBS
+        also[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_seconds)
 
     sweep_gbs = head["sweep_gbs"]

@@ -83,6 +83,27 @@ ts_status ts_sort_by_key(ts_ctx* ctx, ts_dtype key_type, void* d_records, size_t
 ts_status ts_apply_order(ts_ctx* ctx, const uint32_t* d_order, size_t order_len, void* d_other, size_t n,
                          size_t elem_bytes, void* stream);
 
+/* Batched segmented sort.  No single reference function: per segment it is
+ * sort_by_key_with_buffer keyed by the element itself (by_key.rs:175-235)
+ * followed by apply_order on the values (by_key.rs:136-144), the GPU form of
+ * chunks_exact_mut(size).for_each(turbosort::sort) (benches/sort_bench.rs:229-250).
+ * Every segment [d_offsets[s], d_offsets[s+1]) of the n keys of `dtype` is
+ * sorted stably in place; the u32 d_vals (optional, may be NULL) move with
+ * their keys.  Elements outside every segment are left alone.
+ * max_segment_len bounds every segment's length, or is 0 when unknown.  When
+ * it is 0 or above ts_segmented_lds_max_len, the call waits once for the
+ * device: segments too long for one workgroup go through the device radix
+ * sort, and invalid offsets (decreasing, or past n) return
+ * TS_ERR_INVALID_ARG.  Otherwise the call is asynchronous and can be captured
+ * in a graph; the caller guarantees the bound and valid offsets. */
+ts_status ts_segmented_sort_pairs(ts_ctx* ctx, ts_dtype dtype, void* d_keys, uint32_t* d_vals, size_t n,
+                                  const uint64_t* d_offsets, size_t n_segments, size_t max_segment_len,
+                                  void* stream);
+
+/* The longest segment sorted inside one workgroup for this key type (with or
+ * without values); longer segments take the device radix sort. */
+size_t ts_segmented_lds_max_len(ts_dtype dtype, int with_values);
+
 /* ---- SortByKeyBuffer (by_key.rs:51-159) as an object ----
  * A context plus a grow-only device order array.  ts_sbk_sort is
  * sort_by_key_with_buffer (lib.rs:196-202); ts_sbk_order is order()

@@ -98,6 +98,28 @@ def apply_order(order_arr: np.ndarray, values: np.ndarray) -> np.ndarray:
     return values[order_arr.astype(np.int64)]
 
 
+def segmented_sort(keys: np.ndarray, offsets: np.ndarray, values=None):
+    """Expected batched segmented sort: (keys, values) as new arrays.
+
+    Every segment ``[offsets[s], offsets[s+1])`` is put in the stable order
+    by radix key -- per segment ``sort_by_key_with_buffer`` keyed by the key
+    itself with ``apply_order`` on the values (by_key.rs:136-235); elements
+    outside ``[offsets[0], offsets[-1])`` keep their place.
+    """
+    keys_out = keys.copy()
+    vals_out = None if values is None else values.copy()
+    offs = np.asarray(offsets, dtype=np.int64)
+    if offs.size < 2:
+        return keys_out, vals_out
+    lo, hi = int(offs[0]), int(offs[-1])
+    seg = np.repeat(np.arange(offs.size - 1, dtype=np.int64), np.diff(offs))
+    perm = np.lexsort((to_radix(keys[lo:hi]), seg))  # stable: segment, then key, then position
+    keys_out[lo:hi] = keys[lo:hi][perm]
+    if values is not None:
+        vals_out[lo:hi] = values[lo:hi][perm]
+    return keys_out, vals_out
+
+
 # --------------------------------------------------------------- generators
 GAMMA = np.uint64(0x9E3779B97F4A7C15)
 
@@ -132,3 +154,27 @@ def u64_zero_extended_u32(seed: int, n: int) -> np.ndarray:
 def unit_f64(bits: np.ndarray) -> np.ndarray:
     """u64 -> uniform double in [0, 1) from the top 53 bits."""
     return (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def segment_lengths(seed: int, n_segments: int, lo: int = 2, hi: int = 16384) -> np.ndarray:
+    """Log-uniform segment lengths: clamp(floor(exp(U(ln lo, ln(hi + 1)))), lo, hi)
+    with U from the SplitMix64 stream with state ``seed``."""
+    u = unit_f64(splitmix64(seed, n_segments))
+    a, b = np.log(float(lo)), np.log(float(hi + 1))
+    return np.clip(np.floor(np.exp(a + u * (b - a))), lo, hi).astype(np.int64)
+
+
+# The key stream of a segmented workload: state seed + 2^32 (so that it differs
+# from the length stream, state seed).
+SEGMENT_KEY_STREAM = 1 << 32
+
+
+def segmented_workload(seed: int, n_segments: int):
+    """(offsets u64, keys u32, values u32 = global index) of the batched
+    segmented workload: log-uniform lengths in [2, 16384], uniform u32 keys."""
+    lens = segment_lengths(seed, n_segments)
+    offsets = np.zeros(n_segments + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum(lens).astype(np.uint64)
+    n = int(offsets[-1])
+    keys = u32_keys(seed + SEGMENT_KEY_STREAM, n)
+    return offsets, keys, np.arange(n, dtype=np.uint32)

@@ -1,0 +1,143 @@
+"""Batched segmented sort on the GPU vs the oracle.
+
+Each segment must equal a stable key-value sort of that segment: keys in
+radix-key order and values moved with them, ties in input order (per segment:
+sort_by_key_with_buffer keyed by the key plus apply_order, by_key.rs:136-235).
+Lengths cover every size class on both sides of its bound, the device-radix
+class above the LDS capacity, empty and one-key segments and elements outside
+every segment.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import turbosort_amd as ts
+from gpu_util import bits, to_dev, to_host
+from oracle import closed_form as cf
+from shapes import float_specials_f32, random_of, rng
+
+pytestmark = pytest.mark.gpu
+
+# Both sides of every class bound (16, 32, 64, 256, 1024, ..., 16384) and the
+# device class beyond the LDS capacity.
+EDGE_LENGTHS = [0, 1, 2, 3, 5, 15, 16, 17, 31, 32, 33, 63, 64, 65, 200, 255, 256, 257, 1000, 1024, 1025, 2047,
+                2048, 2049, 4095, 4096, 4097, 8191, 8192, 8193, 12000, 16383, 16384, 16385, 40000]
+
+
+def offsets_of(lengths, lead: int = 0) -> np.ndarray:
+    o = np.zeros(len(lengths) + 1, dtype=np.uint64)
+    o[0] = lead
+    o[1:] = lead + np.cumsum(np.asarray(lengths, dtype=np.int64)).astype(np.uint64)
+    return o
+
+
+def run(keys: np.ndarray, offsets: np.ndarray, values, dev, max_len: int = 0):
+    tk = to_dev(keys, dev)
+    to = to_dev(offsets, dev)
+    tv = None if values is None else to_dev(values, dev)
+    ts.segmented_sort(tk, to, tv, max_segment_len=max_len)
+    torch.cuda.synchronize()
+    return to_host(tk, keys.dtype), None if tv is None else to_host(tv, values.dtype)
+
+
+def check(keys, offsets, values, dev, max_len=0):
+    got_k, got_v = run(keys, offsets, values, dev, max_len)
+    want_k, want_v = cf.segmented_sort(keys, offsets, values)
+    assert np.array_equal(bits(got_k), bits(want_k))
+    if values is not None:
+        assert np.array_equal(got_v, want_v)
+
+
+@pytest.mark.parametrize("dtype", cf.DTYPES)
+@pytest.mark.parametrize("with_values", [True, False])
+def test_every_class_every_dtype(gpu, dtype, with_values):
+    lens = list(EDGE_LENGTHS)
+    rng(3).shuffle(lens)
+    off = offsets_of(lens, lead=7)
+    n = int(off[-1]) + 5  # untouched elements before and after the segments
+    keys = random_of(dtype, n, seed=len(dtype) * 7 + int(with_values))
+    vals = np.arange(n, dtype=np.uint32) if with_values else None
+    check(keys, off, vals, gpu)
+
+
+@pytest.mark.parametrize("dtype", ["u32", "f32", "u64", "i16"])
+def test_bounded_length_is_asynchronous_and_equal(gpu, dtype):
+    cap = ts.segmented_lds_max_len(getattr(torch, {"u32": "uint32", "f32": "float32", "u64": "uint64",
+                                                   "i16": "int16"}[dtype]), True)
+    lens = [x for x in EDGE_LENGTHS if x <= cap]
+    off = offsets_of(lens)
+    n = int(off[-1])
+    keys = random_of(dtype, n, seed=11)
+    check(keys, off, np.arange(n, dtype=np.uint32), gpu, max_len=max(lens))
+
+
+@pytest.mark.parametrize("shape", ["few unique", "constant", "sorted", "reversed", "one digit varies"])
+def test_ties_keep_input_order(gpu, shape):
+    lens = [2, 9, 30, 64, 100, 700, 3000, 9000, 16384, 20000]
+    off = offsets_of(lens)
+    n = int(off[-1])
+    i = np.arange(n, dtype=np.uint64)
+    keys = {
+        "few unique": (i % 3).astype(np.uint32),
+        "constant": np.full(n, 0xDEADBEEF, dtype=np.uint32),
+        "sorted": i.astype(np.uint32),
+        "reversed": (n - i).astype(np.uint32),
+        "one digit varies": (0x12345600 | (i * 7 % 16) << np.uint64(4)).astype(np.uint32),
+    }[shape]
+    check(keys, off, np.arange(n, dtype=np.uint32), gpu)
+
+
+def test_float_specials(gpu):
+    lens = [3, 17, 64, 500, 5000, 16384, 17000]
+    off = offsets_of(lens)
+    n = int(off[-1])
+    keys = float_specials_f32(n, seed=5)
+    check(keys, off, np.arange(n, dtype=np.uint32), gpu)
+    check(keys.astype(np.float64), off, np.arange(n, dtype=np.uint32), gpu)
+
+
+@pytest.mark.parametrize("n_segments", [1, 100, 3000])
+def test_log_uniform_workload(gpu, n_segments):
+    off, keys, vals = cf.segmented_workload(7, n_segments)
+    check(keys, off, vals, gpu, max_len=16384)
+
+
+def test_invalid_offsets_raise(gpu):
+    keys = to_dev(np.arange(100, dtype=np.uint32), gpu)
+    bad = to_dev(np.array([0, 50, 40, 100], dtype=np.uint64), gpu)
+    with pytest.raises(ValueError, match="non-decreasing"):
+        ts.segmented_sort(keys, bad)
+    past = to_dev(np.array([0, 50, 101], dtype=np.uint64), gpu)
+    with pytest.raises(ValueError, match="non-decreasing"):
+        ts.segmented_sort(keys, past)
+
+
+def test_full_size_properties(gpu):
+    """10^5 log-uniform segments (1.82e8 pairs) with values = global index.
+
+    Output is correct iff every value stays in its segment, the values form a
+    permutation, each key is the input key at its value's position, keys do
+    not decrease inside a segment and equal keys keep increasing values.
+    """
+    off, keys, _ = cf.segmented_workload(7, 100_000)
+    n = int(off[-1])
+    k = to_dev(keys, gpu)
+    orig = k.view(torch.int32).clone()
+    v = torch.arange(n, dtype=torch.int32, device=gpu)
+    to = to_dev(off, gpu)
+    ts.segmented_sort(k, to, v, max_segment_len=16384)
+    torch.cuda.synchronize()
+    lens = torch.from_numpy(np.diff(off.astype(np.int64))).to(gpu)
+    seg = torch.repeat_interleave(torch.arange(lens.numel(), device=gpu), lens)
+    vl = v.long()
+    assert torch.equal(seg[vl], seg)
+    seen = torch.zeros(n, dtype=torch.bool, device=gpu)
+    seen[vl] = True
+    assert bool(seen.all())
+    assert torch.equal(orig[vl], k.view(torch.int32))
+    kk = k.view(torch.int32).long() & 0xFFFFFFFF
+    same = seg[1:] == seg[:-1]
+    assert not bool((same & (kk[1:] < kk[:-1])).any())
+    assert not bool((same & (kk[1:] == kk[:-1]) & (vl[1:] <= vl[:-1])).any())

@@ -227,3 +227,35 @@ def test_restated_order_matches_stable_argsort(name):
     assert np.array_equal(nv.order(asc), np.arange(1000, dtype=np.uint32))
     desc = np.ascontiguousarray(asc[::-1])
     assert np.array_equal(nv.order(desc), cf.order(desc))
+
+
+def test_segmented_oracle_matches_restated_order_per_segment():
+    """The batched oracle equals the restated by-key order (by_key.rs:175-310)
+    applied segment by segment, values moved by apply_order."""
+    g = np.random.default_rng(21)
+    for name in ("u32", "f32", "i64", "u8"):
+        lens = [0, 1, 2, 5, 16, 17, 100, 129, 1000, 3000]
+        g.shuffle(lens)
+        off = np.concatenate([[3], 3 + np.cumsum(lens)]).astype(np.uint64)
+        n = int(off[-1]) + 4
+        keys = random_of(name, n, seed=len(lens) + n)
+        keys[10:400] = keys[g.integers(10, 400, size=390)]  # ties: stability shows
+        vals = np.arange(n, dtype=np.uint32)
+        got_k, got_v = cf.segmented_sort(keys, off, vals)
+        want_k, want_v = keys.copy(), vals.copy()
+        for lo, hi in zip(off[:-1].astype(np.int64), off[1:].astype(np.int64)):
+            o = nv.order(np.ascontiguousarray(keys[lo:hi]))
+            want_k[lo:hi] = nv.apply_order(o, keys[lo:hi])
+            want_v[lo:hi] = nv.apply_order(o, vals[lo:hi])
+        assert np.array_equal(bits_of(got_k), bits_of(want_k)), name
+        assert np.array_equal(got_v, want_v), name
+
+
+def test_segment_lengths_are_log_uniform_in_range():
+    lens = cf.segment_lengths(7, 100_000)
+    assert lens.min() >= 2 and lens.max() <= 16384
+    # log-uniform: each of the 13 octaves [2^k, 2^(k+1)) holds about 1/13 of the segments
+    octave = np.floor(np.log2(lens)).astype(int)
+    share = np.bincount(octave, minlength=15)[1:14] / lens.size
+    assert np.all(np.abs(share - 1 / 13) < 0.01), share
+    assert abs(lens.mean() - (16385 - 2) / np.log(16385 / 2)) < 40

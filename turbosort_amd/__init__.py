@@ -9,6 +9,8 @@ and by_key.rs:51-159) over device-resident torch tensors:
 * ``sort_by_key_with_buffer(records, buffer, key)`` -- lib.rs:196
 * ``SortByKeyBuffer`` with ``new``, ``with_capacity``, ``capacity``,
   ``order`` and ``apply_order``   -- by_key.rs:51-159
+* ``segmented_sort(keys, offsets, values)`` -- a batched form with no single
+  reference function: a stable key-value sort of every segment
 
 A tensor stands for a slice: its first dimension indexes the elements.  The
 ten sealed key types (key.rs:64-77) are torch.uint8/16/32/64, int8/16/32/64,
@@ -297,6 +299,44 @@ def sort_by_key_with_buffer(records: torch.Tensor, buffer: SortByKeyBuffer, key:
 def sort_by_key(records: torch.Tensor, key: KeyLike = None) -> None:
     """Stable sort by a key with a fresh buffer (lib.rs:152-158)."""
     sort_by_key_with_buffer(records, SortByKeyBuffer(records.device), key)
+
+
+def segmented_sort(keys: torch.Tensor, offsets: torch.Tensor, values: Optional[torch.Tensor] = None, *,
+                   max_segment_len: int = 0, ctx: Optional[Context] = None) -> None:
+    """Sort every segment ``keys[offsets[s]:offsets[s+1]]`` stably, in place.
+
+    ``values`` (u32 or i32, same length as ``keys``) move with their keys.  Per
+    segment this equals ``sort_by_key_with_buffer`` keyed by the key itself
+    plus ``apply_order`` on the values (by_key.rs:136-235).  ``offsets`` holds
+    n_segments + 1 non-decreasing u64 (or i64) positions on the keys' device.
+    ``max_segment_len`` bounds the segment lengths (0: unknown); a bound no
+    larger than ``segmented_lds_max_len`` keeps the call free of host waits.
+    """
+    _slice(keys, "keys")
+    _slice(offsets, "offsets")
+    code = _dtype_code(keys.dtype)
+    if offsets.dtype not in (torch.uint64, torch.int64) or offsets.dim() != 1:
+        raise TypeError("offsets must be a 1-D u64 or i64 tensor")
+    if offsets.device != keys.device:
+        raise ValueError("offsets must live on the keys' device")
+    vp = None
+    if values is not None:
+        _slice(values, "values")
+        if values.dtype not in (torch.uint32, torch.int32):
+            raise TypeError("values must be u32 or i32")
+        if values.numel() != keys.numel() or values.device != keys.device:
+            raise ValueError("values must match the keys' length and device")
+        vp = ctypes.c_void_p(values.data_ptr())
+    nseg = max(offsets.numel() - 1, 0)
+    ctx = ctx or _default_context(keys.device)
+    ctx.check(ctx._lib.ts_segmented_sort_pairs(ctx.handle, code, ctypes.c_void_p(keys.data_ptr()), vp, keys.numel(),
+                                               ctypes.c_void_p(offsets.data_ptr()), nseg, int(max_segment_len),
+                                               _stream(keys.device)))
+
+
+def segmented_lds_max_len(dtype: torch.dtype, with_values: bool = True) -> int:
+    """Longest segment one workgroup sorts for this key type."""
+    return int(_native.load().ts_segmented_lds_max_len(_dtype_code(dtype), int(bool(with_values))))
 
 
 def fill_splitmix64(out: torch.Tensor, seed: int, *, ctx: Optional[Context] = None) -> None:

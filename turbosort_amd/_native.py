@@ -17,7 +17,7 @@ EXPORTS = [
     "ts_status_string", "ts_sort", "ts_sort_with_buffer", "ts_sort_by_key", "ts_apply_order",
     "ts_fill_splitmix64", "ts_checksum", "ts_ctx_set_profiling", "ts_ctx_kernel_times",
     "ts_sbk_create", "ts_sbk_destroy", "ts_sbk_capacity", "ts_sbk_order", "ts_sbk_sort", "ts_sbk_apply_order",
-    "ts_sbk_last_error",
+    "ts_sbk_last_error", "ts_segmented_sort_pairs", "ts_segmented_lds_max_len",
 ]
 
 TS_OK = 0
@@ -63,6 +63,10 @@ def load() -> ctypes.CDLL:
     L.ts_sort_by_key.restype = i32
     L.ts_apply_order.argtypes = [vp, vp, sz, vp, sz, sz, vp]
     L.ts_apply_order.restype = i32
+    L.ts_segmented_sort_pairs.argtypes = [vp, i32, vp, vp, sz, vp, sz, sz, vp]
+    L.ts_segmented_sort_pairs.restype = i32
+    L.ts_segmented_lds_max_len.argtypes = [i32, i32]
+    L.ts_segmented_lds_max_len.restype = sz
     L.ts_fill_splitmix64.argtypes = [vp, vp, sz, u64, i32, vp]
     L.ts_fill_splitmix64.restype = i32
     L.ts_checksum.argtypes = [vp, i32, vp, sz, ctypes.POINTER(u64), ctypes.POINTER(u64), vp]

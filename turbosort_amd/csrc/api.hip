@@ -8,9 +8,11 @@
 // radix sort produces the same unique output for those inputs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/tsort.h"
 #include "radix.h"
@@ -115,6 +117,59 @@ ts_status sort_impl(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, hip
   j.num_cus = c->num_cus;
   hipError_t e = ts::radix_sort(j, s, &c->clk);
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort");
+}
+
+// A segment too long for one workgroup: the device radix sort by key with
+// its order as values, then keys and values gathered into that order.
+ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint32_t* vals, size_t m,
+                          hipStream_t s) {
+  const size_t gbytes = std::max(m * size_t(kb), vals ? m * 4 : size_t(0));
+  ts_status st = ensure(c, by_key_ws(m, kb, gbytes) + up(m * 4));
+  if (st != TS_OK) return st;
+  char* p = c->ws;
+  hipError_t e;
+  uint32_t* order;
+  char* gbuf;
+  if (m <= ts::kSmallMax) {
+    order = reinterpret_cast<uint32_t*>(p);
+    gbuf = p + up(m * 4);
+    e = ts::small_sort(kb, mode, nullptr, m, keys, size_t(kb), 0, order, s, &c->clk);
+    if (e != hipSuccess) return hip_fail(c, e, "small_sort");
+  } else {
+    ts::RadixJob j;
+    j.key_bytes = kb;
+    j.mode = mode;
+    j.n = m;
+    j.records = keys;
+    j.stride = size_t(kb);
+    j.key_offset = 0;
+    j.meta = reinterpret_cast<ts::SortMeta*>(p);
+    p += up(sizeof(ts::SortMeta));
+    j.status = p;
+    p += up(ts::status_bytes(m, kb));
+    j.keys_a = p;
+    p += up(m * size_t(kb));
+    j.keys_b = p;
+    p += up(m * size_t(kb));
+    j.vals_a = reinterpret_cast<uint32_t*>(p);
+    p += up(m * 4);
+    j.vals_b = reinterpret_cast<uint32_t*>(p);
+    p += up(m * 4);
+    order = reinterpret_cast<uint32_t*>(p);
+    p += up(m * 4);
+    gbuf = p;
+    j.final_vals = order;
+    j.num_cus = c->num_cus;
+    e = ts::radix_sort(j, s, &c->clk);
+    if (e != hipSuccess) return hip_fail(c, e, "radix_sort");
+  }
+  e = ts::gather(order, keys, gbuf, m, size_t(kb), s, &c->clk);
+  if (e == hipSuccess) e = hipMemcpyAsync(keys, gbuf, m * size_t(kb), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess && vals) {
+    e = ts::gather(order, vals, gbuf, m, 4, s, &c->clk);
+    if (e == hipSuccess) e = hipMemcpyAsync(vals, gbuf, m * 4, hipMemcpyDeviceToDevice, s);
+  }
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "segment gather");
 }
 
 }  // namespace
@@ -273,6 +328,68 @@ ts_status ts_apply_order(ts_ctx* c, const uint32_t* order, size_t order_len, voi
   if (e != hipSuccess) return hip_fail(c, e, "gather");
   e = hipMemcpyAsync(other, c->ws, n * elem_bytes, hipMemcpyDeviceToDevice, s);
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "hipMemcpyAsync");
+}
+
+ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* vals, size_t n,
+                                  const uint64_t* offsets, size_t n_segments, size_t max_segment_len,
+                                  void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(dt)) return fail(c, TS_ERR_INVALID_ARG, "unknown dtype");
+  clock_reset(c);
+  if (n_segments == 0) return TS_OK;
+  if (!offsets) return fail(c, TS_ERR_INVALID_ARG, "null offsets");
+  if (n_segments >= size_t(UINT32_MAX)) return fail(c, TS_ERR_TOO_LONG, "more than u32::MAX - 1 segments");
+  const int kb = ts::dtype_bytes(dt), mode = ts::dtype_mode(dt);
+  if (n > 0 && !keys) return fail(c, TS_ERR_INVALID_ARG, "null keys");
+  if (reinterpret_cast<uintptr_t>(keys) % size_t(kb) != 0 || reinterpret_cast<uintptr_t>(vals) % 4 != 0)
+    return fail(c, TS_ERR_INVALID_ARG, "keys and values must be aligned to their widths");
+  DeviceScope ds(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t count_bytes = up((ts::kSegClasses + 2) * sizeof(uint32_t));
+  ts_status st = ensure(c, count_bytes + up(size_t(ts::kSegClasses) * n_segments * sizeof(uint32_t)));
+  if (st != TS_OK) return st;
+  ts::SegJob j;
+  j.key_bytes = kb;
+  j.mode = mode;
+  j.keys = keys;
+  j.vals = vals;
+  j.n = n;
+  j.offsets = reinterpret_cast<const unsigned long long*>(offsets);
+  j.n_segments = n_segments;
+  j.counts = reinterpret_cast<uint32_t*>(c->ws);
+  j.lists = reinterpret_cast<uint32_t*>(c->ws + count_bytes);
+  j.num_cus = c->num_cus;
+  hipError_t e = ts::segmented_sort(j, s, &c->clk);
+  if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
+  if (max_segment_len != 0 && max_segment_len <= ts::seg_lds_max_len(kb, vals != nullptr)) return TS_OK;
+
+  // Unknown or long segments: read the class counts back once.
+  uint32_t h[ts::kSegClasses + 2];
+  e = hipMemcpyAsync(h, j.counts, sizeof h, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(c, e, "segment classes");
+  if (h[ts::kSegClasses]) return fail(c, TS_ERR_INVALID_ARG, "segment offsets must be non-decreasing and at most n");
+  const uint32_t nbig = h[ts::kSegDevice];
+  if (nbig == 0) return TS_OK;
+  std::vector<uint32_t> ids(nbig);
+  e = hipMemcpy(ids.data(), j.lists + size_t(ts::kSegDevice) * n_segments, nbig * sizeof(uint32_t),
+                hipMemcpyDeviceToHost);
+  std::vector<uint64_t> bounds(2 * size_t(nbig));
+  for (uint32_t i = 0; i < nbig && e == hipSuccess; ++i)
+    e = hipMemcpy(&bounds[2 * size_t(i)], offsets + ids[i], 2 * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "segment bounds");
+  for (uint32_t i = 0; i < nbig; ++i) {
+    const uint64_t lo = bounds[2 * size_t(i)], m = bounds[2 * size_t(i) + 1] - lo;
+    st = seg_device_sort(c, kb, mode, static_cast<unsigned char*>(keys) + lo * size_t(kb), vals ? vals + lo : nullptr,
+                         size_t(m), s);
+    if (st != TS_OK) return st;
+  }
+  return TS_OK;
+}
+
+size_t ts_segmented_lds_max_len(ts_dtype dt, int with_values) {
+  if (!valid_dtype(dt)) return 0;
+  return ts::seg_lds_max_len(ts::dtype_bytes(dt), with_values != 0);
 }
 
 }  // extern "C"

@@ -22,11 +22,11 @@ struct KernelClock {
     if (!on || count >= kMax) return;
     name[count] = nm;
     bytes[count] = b;
-    hipEventRecord(ev[2 * count], s);
+    (void)hipEventRecord(ev[2 * count], s);
   }
   void end(hipStream_t s) {
     if (!on || count >= kMax) return;
-    hipEventRecord(ev[2 * count + 1], s);
+    (void)hipEventRecord(ev[2 * count + 1], s);
     ++count;
   }
 };
@@ -108,6 +108,29 @@ hipError_t small_sort(int key_bytes, int mode, void* keys, size_t n, const unsig
 // out[i] = src[order[i]] for n elements of elem_bytes bytes.
 hipError_t gather(const uint32_t* order, const void* src, void* dst, size_t n, size_t elem_bytes, hipStream_t s,
                   KernelClock* clk);
+
+// Batched segmented sort (segmented.hip).  Segments of up to
+// seg_lds_max_len keys are sorted inside one workgroup each; longer ones are
+// listed in class kSegDevice (count in counts[kSegDevice], ids from
+// lists + kSegDevice * n_segments) for the caller to sort with the device
+// radix sort.  counts[kSegClasses] becomes non-zero when an offset pair is
+// decreasing or past n.
+constexpr int kSegClasses = 10;
+constexpr int kSegDevice = kSegClasses - 1;
+struct SegJob {
+  int key_bytes = 4;
+  int mode = kIdentity;
+  void* keys = nullptr;
+  uint32_t* vals = nullptr;  // null: keys only
+  size_t n = 0;
+  const unsigned long long* offsets = nullptr;  // n_segments + 1 entries
+  size_t n_segments = 0;
+  uint32_t* counts = nullptr;  // kSegClasses + 2 words
+  uint32_t* lists = nullptr;   // kSegClasses * n_segments words
+  int num_cus = 256;
+};
+uint32_t seg_lds_max_len(int key_bytes, bool vals);
+hipError_t segmented_sort(const SegJob& job, hipStream_t s, KernelClock* clk);
 
 hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStream_t s);
 hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, SortMeta* meta, hipStream_t s);

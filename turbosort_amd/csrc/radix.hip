@@ -15,22 +15,11 @@
 
 #include <algorithm>
 
+#include "device.h"
 #include "radix.h"
 
 namespace ts {
 namespace {
-
-__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
-
-template <class T> __device__ __forceinline__ T wave_inclusive_add(T v) {
-  const unsigned lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    T o = __shfl_up(v, d, 64);
-    if (lane >= unsigned(d)) v += o;
-  }
-  return v;
-}
 
 // Exclusive scan of one value per lane over the first 256 lanes of the
 // workgroup (every lane of the workgroup must call it; s_wave holds one slot
@@ -58,28 +47,6 @@ template <class U> __device__ __forceinline__ U first_lane(U x) {
 
 // Emits nothing; the compiler can no longer see x's value, so values it
 // derived from x earlier are recomputed rather than kept live.
-template <class U> __device__ __forceinline__ void opaque(U& x) {
-  if constexpr (sizeof(U) == 8) {
-    uint64_t t = uint64_t(x);
-    asm("" : "+v"(t));
-    x = U(t);
-  } else {
-    uint32_t t = uint32_t(x);
-    asm("" : "+v"(t));
-    x = U(t);
-  }
-}
-
-template <class U, int N> __device__ __forceinline__ void to_keys(U (&k)[N], int mode) {
-  if (mode == kSigned) {
-#pragma unroll
-    for (int j = 0; j < N; ++j) k[j] = to_key(k[j], kSigned);
-  } else if (mode == kFloat) {
-#pragma unroll
-    for (int j = 0; j < N; ++j) k[j] = to_key(k[j], kFloat);
-  }
-}
-
 // ------------------------------------------------------------------ init
 __global__ void k_init(SortMeta* m, int P) {
   for (unsigned i = threadIdx.x; i < unsigned(P) * kSegments * 256; i += blockDim.x) (&m->pair[0][0][0])[i] = 0;

@@ -1,0 +1,508 @@
+// Batched segmented sort: every segment [offsets[s], offsets[s+1]) of a key
+// array is sorted stably by key on its own, u32 values moving with their keys.
+// Per segment this is sort_by_key_with_buffer keyed by the element itself
+// (by_key.rs:175-235) followed by apply_order on the values
+// (by_key.rs:136-144); the closest reference usage is
+// chunks_exact_mut(size).for_each(turbosort::sort) (benches/sort_bench.rs:229-250).
+//
+// A binning kernel sorts the segment ids into size classes on the device; each
+// class then runs one persistent kernel over its list:
+//   * up to 64 keys: a bitonic network across the lanes of one wave (two or
+//     four segments per wave below 33 and 17 keys) on (key, position) pairs.
+//     The pairs are unique, so the network's result is the stable order.
+//   * up to the LDS capacity of one workgroup: an LSD radix sort inside LDS
+//     with 4-bit digits, skipping every digit on which the segment's keys
+//     agree.  Each lane holds a run of consecutive keys; counts of earlier
+//     equal digits come from 16 nibble counters packed in one 64-bit
+//     register, and a workgroup scan over (digit, lane) counters gives each
+//     key its stable destination.
+//   * longer segments are left in the device class for the caller, which
+//     sorts them with the device-wide radix sort.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "device.h"
+#include "radix.h"
+
+namespace ts {
+namespace {
+
+struct SegArgs {
+  void* keys;
+  uint32_t* vals;  // null: keys only
+  const unsigned long long* offsets;
+  size_t n;
+  size_t n_segments;
+  uint32_t* counts;  // per class, then the invalid-offsets flag
+  uint32_t* lists;   // [kSegClasses][n_segments] segment ids
+  uint32_t caps[kSegClasses];
+  int mode;
+};
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+template <class U> __device__ __forceinline__ U wave_or(U v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) v |= __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Zeroes the class counters (a kernel rather than a memset so that the whole
+// sort can be captured in a graph).
+__global__ void k_seg_clear(uint32_t* counts) {
+  if (threadIdx.x < kSegClasses + 2) counts[threadIdx.x] = 0;
+}
+
+// Appends every segment of two or more keys to its class list.  A workgroup
+// takes kClassifyItems segments per lane, counts each class in LDS (a wave
+// adds all its segments of one class with one LDS atomic) and reserves its
+// slots in the global lists with one atomic per class.
+constexpr int kClassifyThreads = 256;
+constexpr int kClassifyItems = 8;
+
+__global__ __launch_bounds__(kClassifyThreads) void k_seg_classify(SegArgs a) {
+  __shared__ uint32_t lcount[kSegClasses], lbase[kSegClasses];
+  const unsigned tid = threadIdx.x, lane = lane_id();
+  if (tid < kSegClasses) lcount[tid] = 0;
+  __syncthreads();
+  const size_t chunk = size_t(blockIdx.x) * kClassifyThreads * kClassifyItems;
+  int cls[kClassifyItems];
+  uint32_t slot[kClassifyItems];
+#pragma unroll
+  for (int r = 0; r < kClassifyItems; ++r) {
+    const size_t s = chunk + size_t(r) * kClassifyThreads + tid;
+    int c = -1;
+    if (s < a.n_segments) {
+      const unsigned long long lo = a.offsets[s], hi = a.offsets[s + 1];
+      if (hi < lo || hi > a.n) {
+        a.counts[kSegClasses] = 1;
+      } else if (hi - lo >= 2) {
+        const unsigned long long len = hi - lo;
+        c = kSegDevice;
+        for (int x = 0; x < kSegDevice; ++x) {
+          if (len <= a.caps[x]) {
+            c = x;
+            break;
+          }
+        }
+      }
+    }
+    cls[r] = c;
+    slot[r] = 0;
+    uint64_t todo = __ballot(c >= 0);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int cl = __shfl(c, leader, 64);
+      const uint64_t m = __ballot(c == cl);
+      uint32_t b = 0;
+      if (int(lane) == leader) b = atomicAdd(&lcount[cl], uint32_t(__popcll(m)));
+      b = __shfl(b, leader, 64);
+      if (c == cl) slot[r] = b + lanes_below(m);
+      todo &= ~m;
+    }
+  }
+  __syncthreads();
+  if (tid < kSegClasses) lbase[tid] = lcount[tid] ? atomicAdd(&a.counts[tid], lcount[tid]) : 0u;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kClassifyItems; ++r) {
+    if (cls[r] >= 0) {
+      const size_t s = chunk + size_t(r) * kClassifyThreads + tid;
+      a.lists[size_t(cls[r]) * a.n_segments + lbase[cls[r]] + slot[r]] = uint32_t(s);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wave bitonic network: W lanes per segment, 64 / W segments per wave.
+
+template <class U, int W, bool VALS> __global__ __launch_bounds__(256) void k_seg_bitonic(SegArgs a, int cls) {
+  constexpr uint32_t G = 64 / W;
+  using C = std::conditional_t<(sizeof(U) < 4), uint32_t, U>;
+  const uint32_t count = a.counts[cls];
+  const uint32_t* __restrict__ list = a.lists + size_t(cls) * a.n_segments;
+  U* __restrict__ keys = static_cast<U*>(a.keys);
+  const unsigned lane = lane_id(), g = lane / W, i = lane % W;
+  const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), waves = gridDim.x * 4;
+  for (uint32_t base = wave * G; base < count; base += waves * G) {
+    const uint32_t job = base + g;
+    size_t lo = 0;
+    uint32_t len = 0;
+    if (job < count) {
+      const uint32_t seg = list[job];
+      lo = a.offsets[seg];
+      len = uint32_t(a.offsets[seg + 1] - lo);
+    }
+    C k = C(U(~U(0)));
+    uint32_t v = 0;
+    if (i < len) {
+      k = C(to_key<U>(keys[lo + i], a.mode));
+      if (VALS) v = a.vals[lo + i];
+    }
+    uint32_t q = i;
+#pragma unroll
+    for (int kk = 2; kk <= W; kk <<= 1) {
+#pragma unroll
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+        const C ok = __shfl_xor(k, j, 64);
+        const uint32_t oq = __shfl_xor(q, j, 64);
+        const bool less = ok < k || (ok == k && oq < q);
+        // The lower lane of an ascending pair and the upper lane of a
+        // descending pair keep the smaller element.
+        const bool keep_min = ((i & kk) == 0) == ((i & j) == 0);
+        if (less == keep_min) {
+          k = ok;
+          q = oq;
+        }
+      }
+    }
+    uint32_t vo = 0;
+    if (VALS) vo = __shfl(v, int(g * W + q), 64);
+    if (i < len) {
+      keys[lo + i] = from_key<U>(U(k), a.mode);
+      if (VALS) a.vals[lo + i] = vo;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS radix sort: one workgroup of T lanes per segment, IPT keys per lane.
+
+template <class U, int T, int IPT, bool VALS> struct SegRadixLds {
+  static constexpr int CAP = T * IPT;
+  // Arrays indexed through lds_pad: one unused slot after every 32, so that
+  // lanes reading runs of 16 or 32 consecutive slots hit distinct banks.
+  alignas(16) U keys[CAP + CAP / 32];
+  alignas(16) uint32_t vals[VALS ? CAP + CAP / 32 : 1];
+  uint32_t cnt[8 * T + 8 * T / 32];  // (digit & 7, lane) counters; digit >> 3 picks the 16-bit half
+  uint32_t wsum[T / 64];
+  uint32_t any[2];
+};
+
+template <class U> __device__ __forceinline__ uint32_t nib(U k, int sh) { return uint32_t(k >> sh) & 15u; }
+
+__device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 5); }
+
+template <class U, int T, int IPT, bool VALS> __global__ __launch_bounds__(T) void k_seg_radix(SegArgs a, int cls) {
+  static_assert(IPT <= 16 || IPT == 32, "nibble counters cover blocks of 16 keys");
+  static_assert(T * IPT < 65536, "per-half counts are 16-bit");
+  constexpr int NW = T / 64;
+  constexpr int NB = IPT > 16 ? 2 : 1;
+  constexpr int BI = IPT > 16 ? 16 : IPT;  // keys per nibble block
+  constexpr int NIB = 2 * int(sizeof(U));
+  __shared__ SegRadixLds<U, T, IPT, VALS> L;
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t count = a.counts[cls];
+  const uint32_t* __restrict__ list = a.lists + size_t(cls) * a.n_segments;
+  // Padded slots: runs of IPT (a divisor of 32) consecutive positions never
+  // cross a pad, and T is a multiple of 32, so every padded index is a base
+  // plus a constant.
+  constexpr uint32_t TP = T + T / 32;             // lds_pad(i + T) - lds_pad(i)
+  const uint32_t sb = lds_pad(tid);               // striped slot j: sb + j * TP
+  const uint32_t bb = lds_pad(tid * IPT);         // blocked slot j: bb + j
+  const uint32_t rb = lds_pad(tid * 8);           // raking slot x: rb + x
+  static_assert(32 % IPT == 0 || IPT == 32, "runs of IPT slots must not cross a pad");
+  static_assert(T % 32 == 0, "");
+
+  for (uint32_t job = blockIdx.x; job < count; job += gridDim.x) {
+    const uint32_t seg = list[job];
+    const size_t lo = a.offsets[seg];
+    const uint32_t len = uint32_t(a.offsets[seg + 1] - lo);
+    U* __restrict__ keys = static_cast<U*>(a.keys) + lo;
+    uint32_t* __restrict__ vals = VALS ? a.vals + lo : nullptr;
+
+    // Striped, coalesced load (indices clamped so that every load is issued
+    // at once); padding keys are the largest key and, placed after every
+    // real key, stay last.
+    {
+      U kin[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) kin[j] = keys[min(uint32_t(j) * T + tid, len - 1)];
+      if (VALS) {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          const uint32_t i = uint32_t(j) * T + tid;
+          L.vals[sb + j * TP] = vals[min(i, len - 1)];
+        }
+      }
+      to_keys(kin, a.mode);
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t i = uint32_t(j) * T + tid;
+        L.keys[sb + j * TP] = i < len ? kin[j] : U(~U(0));
+      }
+    }
+    if (tid == 0) {
+      L.any[0] = 0;
+      L.any[1] = 0;
+    }
+    __syncthreads();
+
+    // Blocked arrangement: lane tid holds positions tid*IPT .. tid*IPT+IPT-1.
+    U k[IPT];
+    uint32_t v[VALS ? IPT : 1];
+    const U k0 = L.keys[0];
+    U vary = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t p = tid * IPT + j;
+      k[j] = L.keys[bb + j];
+      if (VALS) v[j] = L.vals[bb + j];
+      if (p < len) vary |= U(k[j] ^ k0);
+    }
+    {
+      const uint64_t x = wave_or<uint64_t>(uint64_t(vary));
+      if (lane == 0 && x) {
+        atomicOr(&L.any[0], uint32_t(x));
+        if (sizeof(U) == 8) atomicOr(&L.any[1], uint32_t(x >> 32));
+      }
+    }
+    __syncthreads();
+    const uint64_t varying = uint64_t(L.any[0]) | (uint64_t(L.any[1]) << 32);
+    // With every key equal the stable order is the input order: nothing moves.
+    if (varying == 0) {
+      __syncthreads();  // every lane has read L.any before the next segment clears it
+      continue;
+    }
+
+#pragma unroll 1
+    for (int p = 0; p < NIB; ++p) {
+      const int sh = 4 * p;
+      if (((varying >> sh) & 15u) == 0) continue;  // every key agrees on this digit
+
+      // Earlier keys of this lane with the same digit (bytes packed four to a
+      // register), and per-digit totals in nibble counters.
+      uint32_t before[(IPT + 3) / 4];
+      uint64_t pk[NB];
+      bool same[NB];
+      uint32_t dsame[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        uint64_t c = 0;
+        uint32_t last = 0;
+#pragma unroll
+        for (int jj = 0; jj < BI; ++jj) {
+          const int j = b * 16 + jj;
+          const uint32_t d = nib(k[j], sh);
+          const uint32_t e = uint32_t(c >> (4 * d)) & 15u;
+          if (j % 4 == 0) before[j / 4] = e;
+          else before[j / 4] |= e << (8 * (j % 4));
+          c += uint64_t(1) << (4 * d);
+          last = e;
+        }
+        pk[b] = c;
+        // A nibble holds at most 15, so sixteen equal digits wrap it; then the
+        // last key saw 15 equal digits before it and the block is that digit.
+        same[b] = BI == 16 && last == 15u;
+        dsame[b] = nib(k[b * 16 + BI - 1], sh);
+      }
+      if (NB == 2) {
+        // Keys of the second block also follow the equal digits of the first.
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+          const int j = 16 + jj;
+          const uint32_t d = nib(k[j], sh);
+          const uint32_t t0 = same[0] ? (d == dsame[0] ? 16u : 0u) : uint32_t(pk[0] >> (4 * d)) & 15u;
+          before[j / 4] += t0 << (8 * (j % 4));
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) opaque(k[j]);  // recompute digits below rather than keep them
+      // Counter words: low half digit dl, high half digit dl + 8.
+#pragma unroll
+      for (int dl = 0; dl < 8; ++dl) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const uint32_t lo16 = uint32_t(pk[b] >> (4 * dl)) & 15u;
+          const uint32_t hi16 = uint32_t(pk[b] >> (4 * dl + 32)) & 15u;
+          const uint32_t fix = (dsame[b] & 7u) == uint32_t(dl) ? (16u << (16 * (dsame[b] >> 3))) : 0u;
+          word += same[b] ? fix : (lo16 | (hi16 << 16));
+        }
+        L.cnt[dl * TP + sb] = word;
+      }
+      __syncthreads();
+
+      // Exclusive scan over the counters in (digit, lane) order: each lane
+      // takes eight consecutive words.  Both halves scan at once; their
+      // totals stay below 2^16, so no carry crosses between them.
+      uint32_t r[8];
+      uint32_t sum = 0;
+#pragma unroll
+      for (int x = 0; x < 8; ++x) {
+        r[x] = L.cnt[rb + x];
+        sum += r[x];
+      }
+      const uint32_t inc = wave_inclusive_add(sum);
+      if (lane == 63) L.wsum[w] = inc;
+      __syncthreads();
+      uint32_t wpre = 0, all = 0;
+#pragma unroll
+      for (int x = 0; x < NW; ++x) {
+        const uint32_t s = L.wsum[x];
+        all += s;
+        wpre += x < int(w) ? s : 0u;
+      }
+      uint32_t run = wpre + inc - sum;
+#pragma unroll
+      for (int x = 0; x < 8; ++x) {
+        L.cnt[rb + x] = run;
+        run += r[x];
+      }
+      const uint32_t below8 = all & 0xFFFFu;  // keys whose digit is below 8
+      __syncthreads();
+
+      // Stable destinations; scatter into LDS, then read back blocked.  The
+      // counter reads of eight keys go out before their writes (a write may
+      // alias a later read for all the compiler knows).
+      constexpr int CH = IPT < 8 ? IPT : 8;
+#pragma unroll
+      for (int j0 = 0; j0 < IPT; j0 += CH) {
+        uint32_t wd[CH];
+#pragma unroll
+        for (int jj = 0; jj < CH; ++jj) wd[jj] = L.cnt[(nib(k[j0 + jj], sh) & 7u) * TP + sb];
+#pragma unroll
+        for (int jj = 0; jj < CH; ++jj) {
+          const int j = j0 + jj;
+          const uint32_t d = nib(k[j], sh);
+          const uint32_t base = (d & 8u) ? (wd[jj] >> 16) + below8 : (wd[jj] & 0xFFFFu);
+          const uint32_t dst = base + ((before[j / 4] >> (8 * (j % 4))) & 0xFFu);
+          L.keys[lds_pad(dst)] = k[j];
+          if (VALS) L.vals[lds_pad(dst)] = v[j];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        k[j] = L.keys[bb + j];
+        if (VALS) v[j] = L.vals[bb + j];
+      }
+    }
+
+    // L.keys / L.vals hold the sorted segment; write it out striped.
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = uint32_t(j) * T + tid;
+      if (i < len) {
+        keys[i] = from_key<U>(L.keys[sb + j * TP], a.mode);
+        if (VALS) vals[i] = L.vals[sb + j * TP];
+      }
+    }
+    __syncthreads();  // the next segment reuses the LDS
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Class table.  Class c takes lengths in (cap[c-1], cap[c]]; a cap of 0 marks
+// a class this key width does not use.
+
+struct ClassShape {
+  int bitonic_w;  // > 0: wave network of this width
+  int threads, ipt;
+};
+
+template <class U, bool VALS> constexpr ClassShape class_shape(int c) {
+  // 8-byte keys with values: 16384 keys would need 192 KB of LDS.
+  constexpr bool wide = sizeof(U) == 8;
+  switch (c) {
+    case 0: return {16, 0, 0};
+    case 1: return {32, 0, 0};
+    case 2: return {64, 0, 0};
+    case 3: return {0, 64, 4};
+    case 4: return {0, 64, 16};
+    case 5: return {0, 128, 16};
+    case 6: return {0, 256, 16};
+    case 7: return {0, 512, 16};
+    case 8: return wide && VALS ? ClassShape{0, 0, 0} : ClassShape{0, 512, 32};
+    default: return {0, 0, 0};
+  }
+}
+
+template <class U, bool VALS> constexpr uint32_t class_cap(int c) {
+  const ClassShape s = class_shape<U, VALS>(c);
+  return s.bitonic_w ? uint32_t(s.bitonic_w) : uint32_t(s.threads * s.ipt);
+}
+
+template <class U, bool VALS, int C>
+void launch_class(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk) {
+  constexpr ClassShape sh = class_shape<U, VALS>(C);
+  if constexpr (sh.bitonic_w > 0) {
+    if (clk) clk->begin(s, "k_seg_bitonic", 0);
+    hipLaunchKernelGGL((k_seg_bitonic<U, sh.bitonic_w, VALS>), dim3(num_cus * 8), dim3(256), 0, s, a, C);
+    if (clk) clk->end(s);
+  } else if constexpr (sh.threads > 0) {
+    constexpr size_t lds = sizeof(SegRadixLds<U, sh.threads, sh.ipt, VALS>);
+    constexpr int by_lds = int((160 * 1024) / lds);
+    constexpr int by_threads = 2048 / sh.threads;
+    constexpr int per_cu = std::max(1, std::min(by_lds, by_threads));
+    if (clk) clk->begin(s, "k_seg_radix", 0);
+    hipLaunchKernelGGL((k_seg_radix<U, sh.threads, sh.ipt, VALS>), dim3(num_cus * per_cu), dim3(sh.threads), 0, s,
+                       a, C);
+    if (clk) clk->end(s);
+  }
+}
+
+template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStream_t s, KernelClock* clk) {
+  SegArgs a;
+  a.keys = j.keys;
+  a.vals = VALS ? j.vals : nullptr;
+  a.offsets = j.offsets;
+  a.n = j.n;
+  a.n_segments = j.n_segments;
+  a.counts = j.counts;
+  a.lists = j.lists;
+  a.mode = j.mode;
+  for (int c = 0; c < kSegClasses; ++c) a.caps[c] = c < kSegDevice ? class_cap<U, VALS>(c) : 0u;
+  hipLaunchKernelGGL(k_seg_clear, dim3(1), dim3(64), 0, s, j.counts);
+  const size_t per_block = size_t(kClassifyThreads) * kClassifyItems;
+  const unsigned cgrid = unsigned((j.n_segments + per_block - 1) / per_block);
+  if (clk) clk->begin(s, "k_seg_classify", 0);
+  hipLaunchKernelGGL(k_seg_classify, dim3(cgrid), dim3(kClassifyThreads), 0, s, a);
+  if (clk) clk->end(s);
+  // Longest segments first, so the short classes fill in behind them.
+  launch_class<U, VALS, 8>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 7>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 6>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 5>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 4>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 3>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 2>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 1>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 0>(a, j.num_cus, s, clk);
+  return hipGetLastError();
+}
+
+template <class U, bool VALS> uint32_t max_cap() {
+  uint32_t m = 0;
+  for (int c = 0; c < kSegDevice; ++c) m = std::max(m, class_cap<U, VALS>(c));
+  return m;
+}
+
+}  // namespace
+
+uint32_t seg_lds_max_len(int key_bytes, bool vals) {
+  switch (key_bytes) {
+    case 1: return vals ? max_cap<uint8_t, true>() : max_cap<uint8_t, false>();
+    case 2: return vals ? max_cap<uint16_t, true>() : max_cap<uint16_t, false>();
+    case 4: return vals ? max_cap<uint32_t, true>() : max_cap<uint32_t, false>();
+    case 8: return vals ? max_cap<uint64_t, true>() : max_cap<uint64_t, false>();
+    default: return 0;
+  }
+}
+
+hipError_t segmented_sort(const SegJob& j, hipStream_t s, KernelClock* clk) {
+  const bool v = j.vals != nullptr;
+  switch (j.key_bytes) {
+    case 1: return v ? run_segmented<uint8_t, true>(j, s, clk) : run_segmented<uint8_t, false>(j, s, clk);
+    case 2: return v ? run_segmented<uint16_t, true>(j, s, clk) : run_segmented<uint16_t, false>(j, s, clk);
+    case 4: return v ? run_segmented<uint32_t, true>(j, s, clk) : run_segmented<uint32_t, false>(j, s, clk);
+    case 8: return v ? run_segmented<uint64_t, true>(j, s, clk) : run_segmented<uint64_t, false>(j, s, clk);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace ts
