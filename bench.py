@@ -177,6 +177,14 @@ def score_records(n: int, seed: int):
     return rec
 
 
+def take(t, idx, chunk: int = 1 << 24):
+    """t[idx] in chunks of 2^24 indices (torch's single-call gather has returned
+    zeros past 2^30 output bytes on this stack)."""
+    import torch
+
+    return torch.cat([t[idx[c:c + chunk]] for c in range(0, idx.numel(), chunk)])
+
+
 def timed(fn, steps: int):
     import torch
 
@@ -215,7 +223,7 @@ def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7):
     seen[v] = True
     kk = k.long() & 0xFFFFFFFF
     same = seg[1:] == seg[:-1]
-    ok = (bool(torch.equal(seg[v], seg)) and bool(seen.all()) and bool(torch.equal(src.view(torch.int32)[v], k))
+    ok = (bool(torch.equal(take(seg, v), seg)) and bool(seen.all()) and bool(torch.equal(take(src.view(torch.int32), v), k))
           and not bool((same & (kk[1:] < kk[:-1])).any())
           and not bool((same & (kk[1:] == kk[:-1]) & (v[1:] <= v[:-1])).any()))
     del ks, vs, seg, seen, kk, same
@@ -254,13 +262,13 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5):
     order = buf.order().view(torch.int32).long()
     sbits = src[:, 4:8].contiguous().view(torch.int32).reshape(-1).long() & 0xFFFFFFFF
     radix = torch.where(sbits >= 0x80000000, sbits ^ 0xFFFFFFFF, sbits | 0x80000000)
-    rk = radix[order]
+    rk = take(radix, order)
     seen = torch.zeros(n, dtype=torch.bool, device=dev)
     seen[order] = True
     ok = (bool(seen.all()) and not bool((rk[1:] < rk[:-1]).any())
           and not bool(((rk[1:] == rk[:-1]) & (order[1:] <= order[:-1])).any())
-          and bool(torch.equal(rs[-1], src[order])) and bool(torch.equal(others[-1].view(torch.int64),
-                                                                       other_src.view(torch.int64)[order])))
+          and bool(torch.equal(rs[-1], take(src, order)))
+          and bool(torch.equal(others[-1].view(torch.int64), take(other_src.view(torch.int64), order))))
     del rs, others, order, sbits, radix, rk, seen, src
     torch.cuda.empty_cache()
     return dict(desc="stable sort_by_key, 10^8 16-byte records {u32 id, f32 score, u64 payload} keyed on the "

@@ -91,9 +91,10 @@ ts_status ts_apply_order(ts_ctx* ctx, const uint32_t* d_order, size_t order_len,
  * sorted stably in place; the u32 d_vals (optional, may be NULL) move with
  * their keys.  Elements outside every segment are left alone.
  * max_segment_len bounds every segment's length, or is 0 when unknown.  When
- * it is 0 or above ts_segmented_lds_max_len, the call waits once for the
+ * it is 0 or above ts_segmented_lds_max_len, the call waits for the
  * device: segments too long for one workgroup go through the device radix
- * sort, and invalid offsets (decreasing, or past n) return
+ * sort (with values: by key, stably; keys only: after three-way splits
+ * around pivot keys), and invalid offsets (decreasing, or past n) return
  * TS_ERR_INVALID_ARG.  Otherwise the call is asynchronous and can be captured
  * in a graph; the caller guarantees the bound and valid offsets. */
 ts_status ts_segmented_sort_pairs(ts_ctx* ctx, ts_dtype dtype, void* d_keys, uint32_t* d_vals, size_t n,
@@ -103,6 +104,47 @@ ts_status ts_segmented_sort_pairs(ts_ctx* ctx, ts_dtype dtype, void* d_keys, uin
 /* The longest segment sorted inside one workgroup for this key type (with or
  * without values); longer segments take the device radix sort. */
 size_t ts_segmented_lds_max_len(ts_dtype dtype, int with_values);
+
+/* Learned partitioner.  No reference counterpart: the output equals
+ * ts_sort's (turbosort::sort, lib.rs:99-101) for 4- and 8-byte key types,
+ * reached by bucketing the keys with a monotone model fitted on the device
+ * to a sample the device radix sort has ranked, then sorting each bucket.
+ * Buckets too long for one workgroup are split three ways around a pivot
+ * key until the parts fit (equal keys are equal bits), with the device radix
+ * sort as the last resort.  Waits for the device (bucket overflow).  cfg may
+ * be NULL (defaults); report may be NULL. */
+typedef struct ts_learned_cfg {
+  uint32_t buckets_log2;     /* number of buckets = 2^buckets_log2 (1..24); default 16 */
+  uint32_t sample_per_mille; /* sample size as a fraction of n; default 10 (1 %) */
+  uint32_t train;            /* 1: fit the model (start from the sample's interval histogram, then Adam on the
+                                mean squared CDF error); 0: uniform knots (the ablation); default 1 */
+  uint32_t iters;            /* optimiser steps; default 100 */
+  float lr;                  /* Adam step size; default 0.02 */
+  uint32_t points;           /* training points taken from the sorted sample; default 16384 */
+} ts_learned_cfg;
+
+typedef struct ts_learned_report {
+  uint64_t sample;           /* sample size */
+  uint32_t buckets;
+  uint32_t overflow_buckets; /* buckets longer than one workgroup sorts */
+  uint32_t split_rounds;     /* three-way split rounds those buckets took */
+  uint64_t overflow_keys;    /* keys in those buckets */
+  uint64_t largest_bucket;   /* keys in the largest bucket */
+  float loss_before;         /* mean squared CDF error on the training points: the histogram start */
+  float loss_after;          /* ... after the optimiser steps (the model used) */
+  float loss_uniform;        /* ... of uniform knots (the untrained model) */
+} ts_learned_report;
+
+void ts_learned_cfg_default(ts_learned_cfg* cfg);
+ts_status ts_learned_sort(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, const ts_learned_cfg* cfg,
+                          ts_learned_report* report, void* stream);
+
+/* One fit of the learned model's piecewise-linear CDF (the training kernel of
+ * ts_learned_sort): T points (u in knot units [0, 1024), target t), 1024
+ * parameters theta (in/out), the gradient of the mean squared error at the
+ * initial theta, the loss before and after `iters` Adam steps. */
+ts_status ts_pwl_fit(ts_ctx* ctx, const float* d_u, const float* d_t, size_t T, float* d_theta, float* d_grad,
+                     float* d_loss2, int iters, float lr, void* stream);
 
 /* ---- SortByKeyBuffer (by_key.rs:51-159) as an object ----
  * A context plus a grow-only device order array.  ts_sbk_sort is

@@ -178,3 +178,39 @@ def segmented_workload(seed: int, n_segments: int):
     n = int(offsets[-1])
     keys = u32_keys(seed + SEGMENT_KEY_STREAM, n)
     return offsets, keys, np.arange(n, dtype=np.uint32)
+
+
+# --------------------------------------------------------------- skewed keys
+def lognormal_mixture_keys(seed: int, n: int, as_f64: bool = False, hot_share: float = 0.05,
+                           n_hot: int = 10_000) -> np.ndarray:
+    """Skewed keys for the learned partitioner.
+
+    x from a mixture of lognormals (mu 0, sigma 0.5 / 1 / 2 with weights
+    0.5 / 0.3 / 0.2); u64 key = floor(min(x / e^12, 1 - 2^-53) * 2^64), or x
+    itself as f64.  A share ``hot_share`` of the positions is replaced by
+    Zipf(s = 1.1) draws over ``n_hot`` hot values from the same mixture.
+    Streams: SplitMix64 states seed + k * 2^32 (k = 0 .. 7).
+    """
+    st = lambda k: seed + k * (1 << 32)  # noqa: E731
+
+    def mixture(n_, k0):
+        u1 = unit_f64(splitmix64(st(k0), n_))
+        u2 = unit_f64(splitmix64(st(k0 + 1), n_))
+        pick = unit_f64(splitmix64(st(k0 + 2), n_))
+        z = np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+        sigma = np.where(pick < 0.5, 0.5, np.where(pick < 0.8, 1.0, 2.0))
+        return np.exp(sigma * z)
+
+    x = mixture(n, 0)
+    hot = mixture(n_hot, 3)
+    w = np.arange(1, n_hot + 1, dtype=np.float64) ** -1.1
+    cdf = np.cumsum(w) / w.sum()
+    rank = np.minimum(np.searchsorted(cdf, unit_f64(splitmix64(st(6), n)), side="right"), n_hot - 1)
+    repl = unit_f64(splitmix64(st(7), n)) < hot_share
+    x = np.where(repl, hot[rank], x)
+    if as_f64:
+        return x
+    v = np.minimum(x / np.exp(12.0), 1.0 - 2.0 ** -53)
+    hi = np.floor(v * 4294967296.0)
+    lo = np.floor((v * 4294967296.0 - hi) * 4294967296.0)
+    return (hi.astype(np.uint64) << np.uint64(32)) | lo.astype(np.uint64)

@@ -1,0 +1,129 @@
+"""Learned partitioner on the GPU: exact output, training kernel, ablation.
+
+learned_sort must give exactly sort()'s output (the unique ascending order of
+the radix keys, lib.rs:99-101) for every input, whatever the model.  The
+training kernel's gradient is checked against the float64 oracle
+(oracle/pwl.py, itself pinned by finite differences), and the fitted model
+must balance buckets better than the untrained (uniform-knot) model.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import turbosort_amd as ts
+from gpu_util import bits, to_dev, to_host
+from oracle import closed_form as cf
+from oracle import pwl
+from shapes import float_specials_f32, random_of, rng, structured_u64
+
+pytestmark = pytest.mark.gpu
+
+
+def check_sorted(x: np.ndarray, dev, config=None) -> dict:
+    t = to_dev(x, dev)
+    rep = ts.learned_sort(t, config=config)
+    torch.cuda.synchronize()
+    assert np.array_equal(bits(to_host(t, x.dtype)), bits(cf.sort(x)))
+    return rep
+
+
+@pytest.mark.parametrize("n", [2, 3, 100, 1025, 5000, 100_003, 1_000_000])
+@pytest.mark.parametrize("dtype", ["u64", "f64", "u32", "i64", "i32", "f32"])
+def test_uniform_bits(gpu, n, dtype):
+    check_sorted(random_of(dtype, n, seed=n + len(dtype)), gpu)
+
+
+@pytest.mark.parametrize("n", [1000, 300_000, 2_000_000])
+def test_lognormal_mixture(gpu, n):
+    rep = check_sorted(cf.lognormal_mixture_keys(8, n), gpu)
+    assert rep["loss_after"] < rep["loss_before"]
+    check_sorted(cf.lognormal_mixture_keys(8, n, as_f64=True), gpu)
+
+
+def test_structured_and_degenerate(gpu):
+    for shape, x in structured_u64(200_000, seed=31).items():
+        check_sorted(x, gpu)
+    n = 150_000
+    i = np.arange(n, dtype=np.uint64)
+    for x in (np.full(n, 7, dtype=np.uint64), i, i[::-1].copy(), (i % 3).astype(np.uint64),
+              np.concatenate([np.zeros(n // 2, np.uint64), np.full(n - n // 2, 2**63, np.uint64)]),
+              (i % 2).astype(np.uint64) * np.uint64(2**64 - 1)):
+        check_sorted(x, gpu)
+    check_sorted(float_specials_f32(n, seed=4), gpu)
+    check_sorted(float_specials_f32(n, seed=5).astype(np.float64), gpu)
+
+
+@pytest.mark.parametrize("buckets_log2", [1, 8, 12, 20])
+@pytest.mark.parametrize("train", [True, False])
+def test_configurations(gpu, buckets_log2, train):
+    x = cf.lognormal_mixture_keys(9, 400_000)
+    cfg = ts.LearnedConfig(buckets_log2=buckets_log2, train=train, iters=50)
+    rep = check_sorted(x, gpu, cfg)
+    assert rep["buckets"] == 1 << buckets_log2
+    assert rep["sample"] == 4000
+
+
+def test_gradient_matches_oracle(gpu):
+    g = rng(12)
+    T = 40_000
+    u = np.sort(g.uniform(0, 1024, T)).astype(np.float32)
+    t = np.sort(g.uniform(0, 1, T)).astype(np.float32)
+    theta = g.normal(0.5, 0.7, 1024).astype(np.float32)
+    th_dev = torch.from_numpy(theta.copy()).to(gpu)
+    grad, l0, l1 = ts.pwl_fit(torch.from_numpy(u).to(gpu), torch.from_numpy(t).to(gpu), th_dev, iters=0)
+    want_l, want_g = pwl.loss_grad(theta, u, t)
+    got = grad.cpu().numpy().astype(np.float64)
+    assert abs(l0 - want_l) <= 1e-5 * max(1.0, want_l) + 1e-7
+    assert l1 == l0  # no step taken
+    # float32 accumulation over 40k points against float64
+    scale = np.abs(want_g).max()
+    assert np.max(np.abs(got - want_g)) <= 2e-3 * scale, (np.max(np.abs(got - want_g)), scale)
+    assert np.array_equal(th_dev.cpu().numpy(), theta)
+
+
+def test_training_lowers_the_loss(gpu):
+    x = np.sort(cf.to_radix(cf.lognormal_mixture_keys(8, 200_000)))
+    # the points the sort trains on: log coordinate over the sample's range
+    lg = np.log2(np.maximum(x.astype(np.float64), 1.0))
+    u = ((lg - lg[0]) / (lg[-1] - lg[0]) * 1023.999).astype(np.float32)
+    t = ((np.arange(x.size) + 0.5) / x.size).astype(np.float32)
+    theta = torch.full((1024,), 0.54132485, dtype=torch.float32, device=gpu)
+    _, l0, l1 = ts.pwl_fit(torch.from_numpy(u).to(gpu), torch.from_numpy(t).to(gpu), theta, iters=300)
+    assert l1 < 0.05 * l0, (l0, l1)
+    # the trained theta gives the oracle the same loss
+    assert abs(pwl.loss(theta.cpu().numpy(), u, t) - l1) < 1e-3 * l0
+
+
+def test_trained_model_balances_buckets_better_than_uniform_knots(gpu):
+    """Ablation: the fitted CDF against the untrained model (equal-width
+    buckets over the sample's log-key range) on the skewed mixture."""
+    x = cf.lognormal_mixture_keys(8, 2_000_000, hot_share=0.0)
+    base = ts.LearnedConfig(buckets_log2=12)
+    trained = check_sorted(x, gpu, base)
+    plain = check_sorted(x, gpu, ts.LearnedConfig(buckets_log2=12, train=False))
+    mean = x.size / 4096
+    assert trained["loss_after"] <= trained["loss_before"] < trained["loss_uniform"], trained
+    assert trained["largest_bucket"] < 4 * mean, trained
+    assert plain["largest_bucket"] > 2 * trained["largest_bucket"], (plain, trained)
+
+
+def test_rejects_other_widths(gpu):
+    t = torch.zeros(10, dtype=torch.uint16, device=gpu)
+    with pytest.raises(ValueError, match="4- and 8-byte"):
+        ts.learned_sort(t)
+
+
+@pytest.mark.parametrize("dtype", ["u64", "f64", "u32"])
+def test_heavy_hitters_take_the_split_path(gpu, dtype):
+    """Buckets far beyond one workgroup (a few values repeated millions of
+    times) go through three-way splits; the output stays exact."""
+    n = 3_000_000
+    g = rng(77)
+    x = random_of(dtype, n, seed=78)
+    hot = random_of(dtype, 3, seed=79)
+    pick = g.random(n)
+    x = np.where(pick < 0.5, hot[0], np.where(pick < 0.7, hot[1], np.where(pick < 0.75, hot[2], x)))
+    rep = check_sorted(x.astype(cf.NP_DTYPES[dtype]), gpu)
+    assert rep["overflow_buckets"] >= 1 and rep["split_rounds"] >= 1, rep

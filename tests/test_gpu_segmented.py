@@ -132,12 +132,31 @@ def test_full_size_properties(gpu):
     lens = torch.from_numpy(np.diff(off.astype(np.int64))).to(gpu)
     seg = torch.repeat_interleave(torch.arange(lens.numel(), device=gpu), lens)
     vl = v.long()
-    assert torch.equal(seg[vl], seg)
+
+    def take(t, idx, chunk=1 << 24):  # torch's one-call gather has returned zeros past 2^30 output bytes
+        return torch.cat([t[idx[c:c + chunk]] for c in range(0, idx.numel(), chunk)])
+
+    assert torch.equal(take(seg, vl), seg)
     seen = torch.zeros(n, dtype=torch.bool, device=gpu)
     seen[vl] = True
     assert bool(seen.all())
-    assert torch.equal(orig[vl], k.view(torch.int32))
+    assert torch.equal(take(orig, vl), k.view(torch.int32))
     kk = k.view(torch.int32).long() & 0xFFFFFFFF
     same = seg[1:] == seg[:-1]
     assert not bool((same & (kk[1:] < kk[:-1])).any())
     assert not bool((same & (kk[1:] == kk[:-1]) & (vl[1:] <= vl[:-1])).any())
+
+
+@pytest.mark.parametrize("dtype", ["u32", "u64", "f32", "i8"])
+def test_long_keys_only_segments_split(gpu, dtype):
+    """Keys-only segments beyond the LDS capacity: repeated values, few
+    distinct values and random keys, sorted through three-way splits."""
+    n_hot = 200_000
+    g = rng(55)
+    a = random_of(dtype, n_hot, seed=56)
+    a[g.random(n_hot) < 0.6] = a[0]
+    b = random_of(dtype, 5, seed=57)[g.integers(0, 5, size=150_000)]
+    c = random_of(dtype, 90_000, seed=58)
+    keys = np.concatenate([a, b, c, random_of(dtype, 300, seed=59)])
+    off = offsets_of([n_hot, 150_000, 90_000, 300])
+    check(keys, off, None, gpu)

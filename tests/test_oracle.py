@@ -259,3 +259,39 @@ def test_segment_lengths_are_log_uniform_in_range():
     share = np.bincount(octave, minlength=15)[1:14] / lens.size
     assert np.all(np.abs(share - 1 / 13) < 0.01), share
     assert abs(lens.mean() - (16385 - 2) / np.log(16385 / 2)) < 40
+
+
+def test_pwl_gradient_oracle_matches_finite_differences():
+    """The learned model's float64 gradient (oracle/pwl.py) against central
+    differences of its loss, on parameters of both signs and on points that
+    hit the first and last intervals."""
+    from oracle import pwl
+
+    g = np.random.default_rng(5)
+    theta = g.normal(0.0, 2.0, pwl.K)
+    u = np.sort(np.concatenate([g.uniform(0, pwl.K, 3000), [0.0, 0.25, pwl.K - 0.5, pwl.K - 1e-3]]))
+    t = np.sort(g.uniform(0, 1, u.size))
+    _, grad = pwl.loss_grad(theta, u, t)
+    for j in (0, 1, 17, 511, 1000, pwl.K - 1):
+        e = 1e-5
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += e
+        tm[j] -= e
+        fd = (pwl.loss(tp, u, t) - pwl.loss(tm, u, t)) / (2 * e)
+        assert abs(fd - grad[j]) <= 1e-6 * max(1e-3, abs(fd)) + 1e-11, (j, fd, grad[j])
+    # knots: monotone, from 0 to 1
+    y = pwl.knots(theta)
+    assert y[0] == 0.0 and abs(y[-1] - 1.0) < 1e-12 and np.all(np.diff(y) >= 0)
+
+
+def test_lognormal_mixture_generator():
+    x = cf.lognormal_mixture_keys(8, 200_000)
+    assert x.dtype == np.uint64
+    assert np.array_equal(x, cf.lognormal_mixture_keys(8, 200_000))
+    # bulk near 2^47 (x ~ 1 -> e^-12 * 2^64), a 5 % hot set with heavy repeats
+    med = np.median(np.log2(x.astype(np.float64)))
+    assert 45.5 < med < 47.5, med
+    _, counts = np.unique(x, return_counts=True)
+    assert counts.max() > 0.005 * x.size
+    f = cf.lognormal_mixture_keys(8, 1000, as_f64=True)
+    assert f.dtype == np.float64 and np.all(f > 0)

@@ -11,6 +11,8 @@ and by_key.rs:51-159) over device-resident torch tensors:
   ``order`` and ``apply_order``   -- by_key.rs:51-159
 * ``segmented_sort(keys, offsets, values)`` -- a batched form with no single
   reference function: a stable key-value sort of every segment
+* ``learned_sort(t)`` -- ``sort``'s output reached through a learned
+  partitioner (no reference counterpart)
 
 A tensor stands for a slice: its first dimension indexes the elements.  The
 ten sealed key types (key.rs:64-77) are torch.uint8/16/32/64, int8/16/32/64,
@@ -337,6 +339,62 @@ def segmented_sort(keys: torch.Tensor, offsets: torch.Tensor, values: Optional[t
 def segmented_lds_max_len(dtype: torch.dtype, with_values: bool = True) -> int:
     """Longest segment one workgroup sorts for this key type."""
     return int(_native.load().ts_segmented_lds_max_len(_dtype_code(dtype), int(bool(with_values))))
+
+
+@dataclass
+class LearnedConfig:
+    """Settings of ``learned_sort`` (ts_learned_cfg)."""
+
+    buckets_log2: int = 16
+    sample_per_mille: int = 10
+    train: bool = True
+    iters: int = 100
+    lr: float = 0.02
+    points: int = 16384
+
+
+def learned_sort(t: torch.Tensor, *, config: Optional[LearnedConfig] = None, ctx: Optional[Context] = None) -> dict:
+    """Sort a 1-D tensor of 4- or 8-byte keys in place through the learned partitioner.
+
+    The result equals ``sort(t)``: a monotone CDF model fitted on the device to
+    a sample buckets the keys, and every bucket is then sorted.  Returns the
+    run's report (sample size, bucket overflow, model loss before and after
+    fitting).
+    """
+    _slice(t)
+    code = _dtype_code(t.dtype)
+    c = config or LearnedConfig()
+    cfg = _native.LearnedCfg(c.buckets_log2, c.sample_per_mille, int(bool(c.train)), c.iters, c.lr, c.points)
+    rep = _native.LearnedReport()
+    ctx = ctx or _default_context(t.device)
+    ctx.check(ctx._lib.ts_learned_sort(ctx.handle, code, ctypes.c_void_p(t.data_ptr()), t.numel(), ctypes.byref(cfg),
+                                       ctypes.byref(rep), _stream(t.device)))
+    return {name: getattr(rep, name) for name, _ in rep._fields_}
+
+
+def pwl_fit(u: torch.Tensor, target: torch.Tensor, theta: torch.Tensor, *, iters: int = 0, lr: float = 0.05,
+            ctx: Optional[Context] = None):
+    """One run of the learned model's training kernel.
+
+    ``u`` (knot units, sorted) and ``target`` are float32 points; ``theta``
+    (1024 float32) is updated in place.  Returns (gradient at the initial
+    theta, loss before, loss after ``iters`` Adam steps).
+    """
+    for x, nm in ((u, "u"), (target, "target"), (theta, "theta")):
+        _slice(x, nm)
+        if x.dtype != torch.float32:
+            raise TypeError(f"{nm} must be float32")
+    if theta.numel() != 1024 or u.numel() != target.numel():
+        raise ValueError("theta has 1024 entries; u and target have equal lengths")
+    grad = torch.empty(1024, dtype=torch.float32, device=theta.device)
+    loss = torch.zeros(2, dtype=torch.float32, device=theta.device)
+    ctx = ctx or _default_context(theta.device)
+    ctx.check(ctx._lib.ts_pwl_fit(ctx.handle, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(target.data_ptr()),
+                                  u.numel(), ctypes.c_void_p(theta.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
+                                  ctypes.c_void_p(loss.data_ptr()), int(iters), float(lr), _stream(theta.device)))
+    torch.cuda.synchronize(theta.device)
+    lh = loss.cpu()
+    return grad, float(lh[0]), float(lh[1])
 
 
 def fill_splitmix64(out: torch.Tensor, seed: int, *, ctx: Optional[Context] = None) -> None:

@@ -18,7 +18,22 @@ EXPORTS = [
     "ts_fill_splitmix64", "ts_checksum", "ts_ctx_set_profiling", "ts_ctx_kernel_times",
     "ts_sbk_create", "ts_sbk_destroy", "ts_sbk_capacity", "ts_sbk_order", "ts_sbk_sort", "ts_sbk_apply_order",
     "ts_sbk_last_error", "ts_segmented_sort_pairs", "ts_segmented_lds_max_len",
+    "ts_learned_cfg_default", "ts_learned_sort", "ts_pwl_fit",
 ]
+
+
+class LearnedCfg(ctypes.Structure):
+    """ts_learned_cfg"""
+    _fields_ = [("buckets_log2", ctypes.c_uint32), ("sample_per_mille", ctypes.c_uint32),
+                ("train", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("lr", ctypes.c_float),
+                ("points", ctypes.c_uint32)]
+
+
+class LearnedReport(ctypes.Structure):
+    """ts_learned_report"""
+    _fields_ = [("sample", ctypes.c_uint64), ("buckets", ctypes.c_uint32), ("overflow_buckets", ctypes.c_uint32),
+                ("split_rounds", ctypes.c_uint32), ("overflow_keys", ctypes.c_uint64), ("largest_bucket", ctypes.c_uint64),
+                ("loss_before", ctypes.c_float), ("loss_after", ctypes.c_float), ("loss_uniform", ctypes.c_float)]
 
 TS_OK = 0
 TS_ERR_BUFFER_TOO_SMALL = 1
@@ -67,6 +82,12 @@ def load() -> ctypes.CDLL:
     L.ts_segmented_sort_pairs.restype = i32
     L.ts_segmented_lds_max_len.argtypes = [i32, i32]
     L.ts_segmented_lds_max_len.restype = sz
+    L.ts_learned_cfg_default.argtypes = [ctypes.POINTER(LearnedCfg)]
+    L.ts_learned_cfg_default.restype = None
+    L.ts_learned_sort.argtypes = [vp, i32, vp, sz, ctypes.POINTER(LearnedCfg), ctypes.POINTER(LearnedReport), vp]
+    L.ts_learned_sort.restype = i32
+    L.ts_pwl_fit.argtypes = [vp, vp, vp, sz, vp, vp, vp, i32, ctypes.c_float, vp]
+    L.ts_pwl_fit.restype = i32
     L.ts_fill_splitmix64.argtypes = [vp, vp, sz, u64, i32, vp]
     L.ts_fill_splitmix64.restype = i32
     L.ts_checksum.argtypes = [vp, i32, vp, sz, ctypes.POINTER(u64), ctypes.POINTER(u64), vp]

@@ -30,7 +30,7 @@ FLAGS = [
     "-I", os.path.join(os.path.dirname(HERE), "include"),
 ]
 
-SOURCES = ["radix.hip", "segmented.hip", "small.hip", "util.hip", "api.hip"]
+SOURCES = ["radix.hip", "segmented.hip", "learned.hip", "small.hip", "util.hip", "api.hip"]
 
 
 def _compile(src: str) -> str:

@@ -123,6 +123,21 @@ ts_status sort_impl(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, hip
 // its order as values, then keys and values gathered into that order.
 ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint32_t* vals, size_t m,
                           hipStream_t s) {
+  if (!vals && m > ts::kSmallMax) {  // keys only: the plain device sort in place
+    ts_status st = ensure(c, sort_ws(m, kb, true));
+    if (st != TS_OK) return st;
+    ts::RadixJob j;
+    j.key_bytes = kb;
+    j.mode = mode;
+    j.n = m;
+    j.meta = reinterpret_cast<ts::SortMeta*>(c->ws);
+    j.status = c->ws + up(sizeof(ts::SortMeta));
+    j.keys_a = keys;
+    j.keys_b = c->ws + up(sizeof(ts::SortMeta)) + up(ts::status_bytes(m, kb));
+    j.num_cus = c->num_cus;
+    hipError_t e = ts::radix_sort(j, s, &c->clk);
+    return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort");
+  }
   const size_t gbytes = std::max(m * size_t(kb), vals ? m * 4 : size_t(0));
   ts_status st = ensure(c, by_key_ws(m, kb, gbytes) + up(m * 4));
   if (st != TS_OK) return st;
@@ -170,6 +185,127 @@ ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint
     if (e == hipSuccess) e = hipMemcpyAsync(vals, gbuf, m * 4, hipMemcpyDeviceToDevice, s);
   }
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "segment gather");
+}
+
+// Device arrays for sort_long_keys: room for `f` long segments per round.
+size_t long_keys_bytes(size_t f) {
+  return 5 * up(f * 8) + up(f * 4) + 2 * up(2 * f * 8) + up((ts::kSegClasses + 2) * 4) +
+         up(size_t(ts::kSegClasses) * 2 * f * 4);
+}
+// At most this many segments longer than `cap` fit in n keys.
+size_t long_keys_max(size_t n, size_t cap) { return n / (cap + 1) + 1; }
+
+// Keys-only segments too long for one workgroup (equal keys are equal bits,
+// so no order among them is observable): rounds of three-way splits around a
+// pivot; the parts that fit one workgroup go to the segmented LDS classes,
+// longer parts are split again.  After kRounds rounds the rest take the
+// device radix sort.  `tmp` holds n keys; `area` has long_keys_bytes(f) bytes
+// with f >= long_keys_max(n, cap).  Returns the rounds used in *rounds.
+ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, size_t n,
+                         std::vector<std::pair<uint64_t, uint64_t>> segs, char* area, size_t f, hipStream_t s,
+                         uint32_t* rounds) {
+  constexpr int kRounds = 12;
+  const uint32_t cap = ts::seg_lds_max_len(kb, false);
+  size_t o = 0;
+  auto region = [&](size_t bytes) {
+    char* r = area + o;
+    o += up(bytes);
+    return r;
+  };
+  auto* d_lo = reinterpret_cast<unsigned long long*>(region(f * 8));
+  auto* d_len = reinterpret_cast<unsigned long long*>(region(f * 8));
+  auto* d_below = reinterpret_cast<unsigned long long*>(region(f * 8));
+  auto* d_above = reinterpret_cast<unsigned long long*>(region(f * 8));
+  auto* d_pivot = reinterpret_cast<unsigned long long*>(region(f * 8));
+  auto* d_chunks = reinterpret_cast<uint32_t*>(region(f * 4));
+  auto* d_starts = reinterpret_cast<unsigned long long*>(region(2 * f * 8));
+  auto* d_ends = reinterpret_cast<unsigned long long*>(region(2 * f * 8));
+  auto* d_counts = reinterpret_cast<uint32_t*>(region((ts::kSegClasses + 2) * 4));
+  auto* d_lists = reinterpret_cast<uint32_t*>(region(size_t(ts::kSegClasses) * 2 * f * 4));
+  const uint32_t chunk = ts::split_chunk();
+  std::vector<std::vector<unsigned long long>> keep;  // host sources of the copies, alive until the end
+  uint32_t r = 0;
+  for (; r < uint32_t(kRounds) && !segs.empty(); ++r) {
+    const size_t ns = segs.size();
+    if (ns > f) return fail(c, TS_ERR_INVALID_ARG, "internal: too many long segments");
+    keep.emplace_back(2 * ns);
+    std::vector<unsigned long long>& lolen = keep.back();
+    std::vector<uint32_t> cs(ns);
+    uint32_t chunks = 0;
+    for (size_t i = 0; i < ns; ++i) {
+      lolen[i] = segs[i].first;
+      lolen[ns + i] = segs[i].second;
+      cs[i] = chunks;
+      chunks += uint32_t((segs[i].second + chunk - 1) / chunk);
+    }
+    hipError_t e = hipMemcpyAsync(d_lo, lolen.data(), ns * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_len, lolen.data() + ns, ns * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_chunks, cs.data(), ns * 4, hipMemcpyHostToDevice, s);
+    ts::SplitJob sj;
+    sj.key_bytes = kb;
+    sj.mode = mode;
+    sj.data = data;
+    sj.tmp = tmp;
+    sj.nseg = uint32_t(ns);
+    sj.chunks = chunks;
+    sj.lo = d_lo;
+    sj.len = d_len;
+    sj.chunk_start = d_chunks;
+    sj.below = d_below;
+    sj.above = d_above;
+    sj.pivot = d_pivot;
+    if (e == hipSuccess) e = ts::split_segments(sj, s);
+    std::vector<unsigned long long> ba(2 * ns);
+    if (e == hipSuccess) e = hipMemcpyAsync(ba.data(), d_below, ns * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ba.data() + ns, d_above, ns * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e, "split");
+    std::vector<std::pair<uint64_t, uint64_t>> next;
+    keep.emplace_back();
+    std::vector<unsigned long long>& se = keep.back();
+    std::vector<unsigned long long> ends;
+    for (size_t i = 0; i < ns; ++i) {
+      const uint64_t lo = segs[i].first, len = segs[i].second, nb = ba[i], na = ba[ns + i];
+      const std::pair<uint64_t, uint64_t> parts[2] = {{lo, nb}, {lo + len - na, na}};
+      for (const auto& pt : parts) {
+        if (pt.second < 2) continue;
+        if (pt.second <= cap) {
+          se.push_back(pt.first);
+          ends.push_back(pt.first + pt.second);
+        } else {
+          next.push_back(pt);
+        }
+      }
+    }
+    if (!se.empty()) {
+      const size_t m = se.size();
+      se.insert(se.end(), ends.begin(), ends.end());
+      e = hipMemcpyAsync(d_starts, se.data(), m * 8, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(d_ends, se.data() + m, m * 8, hipMemcpyHostToDevice, s);
+      ts::SegJob gj;
+      gj.key_bytes = kb;
+      gj.mode = mode;
+      gj.keys = data;
+      gj.n = n;
+      gj.offsets = d_starts;
+      gj.ends = d_ends;
+      gj.n_segments = m;
+      gj.counts = d_counts;
+      gj.lists = d_lists;
+      gj.num_cus = c->num_cus;
+      if (e == hipSuccess) e = ts::segmented_sort(gj, s, &c->clk);
+      if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
+      // The next round reuses d_starts / d_ends only after this sort (same stream).
+    }
+    segs.swap(next);
+  }
+  if (rounds) *rounds = r;
+  for (const auto& sg : segs) {
+    ts_status st = seg_device_sort(c, kb, mode, static_cast<unsigned char*>(data) + sg.first * size_t(kb), nullptr,
+                                   size_t(sg.second), s);
+    if (st != TS_OK) return st;
+  }
+  return TS_OK;
 }
 
 }  // namespace
@@ -378,6 +514,16 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
   for (uint32_t i = 0; i < nbig && e == hipSuccess; ++i)
     e = hipMemcpy(&bounds[2 * size_t(i)], offsets + ids[i], 2 * sizeof(uint64_t), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(c, e, "segment bounds");
+  if (!vals) {
+    // Keys only: three-way splits, then the LDS classes (the workspace's
+    // class lists are no longer needed, so it may be regrown).
+    const size_t fmax = long_keys_max(n, ts::seg_lds_max_len(kb, false));
+    st = ensure(c, up(n * size_t(kb)) + long_keys_bytes(fmax));
+    if (st != TS_OK) return st;
+    std::vector<std::pair<uint64_t, uint64_t>> fat;
+    for (uint32_t i = 0; i < nbig; ++i) fat.emplace_back(bounds[2 * size_t(i)], bounds[2 * size_t(i) + 1] - bounds[2 * size_t(i)]);
+    return sort_long_keys(c, kb, mode, keys, c->ws, n, std::move(fat), c->ws + up(n * size_t(kb)), fmax, s, nullptr);
+  }
   for (uint32_t i = 0; i < nbig; ++i) {
     const uint64_t lo = bounds[2 * size_t(i)], m = bounds[2 * size_t(i) + 1] - lo;
     st = seg_device_sort(c, kb, mode, static_cast<unsigned char*>(keys) + lo * size_t(kb), vals ? vals + lo : nullptr,
@@ -385,6 +531,159 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
     if (st != TS_OK) return st;
   }
   return TS_OK;
+}
+
+void ts_learned_cfg_default(ts_learned_cfg* cfg) {
+  if (!cfg) return;
+  cfg->buckets_log2 = 16;
+  cfg->sample_per_mille = 10;
+  cfg->train = 1;
+  cfg->iters = 100;
+  cfg->lr = 0.02f;
+  cfg->points = 16384;
+}
+
+ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts_learned_cfg* cfg_in,
+                          ts_learned_report* report, void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(dt)) return fail(c, TS_ERR_INVALID_ARG, "unknown dtype");
+  const int kb = ts::dtype_bytes(dt), mode = ts::dtype_mode(dt);
+  if (kb != 4 && kb != 8) return fail(c, TS_ERR_INVALID_ARG, "the learned sort takes 4- and 8-byte keys");
+  ts_learned_cfg cfg;
+  ts_learned_cfg_default(&cfg);
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.buckets_log2 < 1 || cfg.buckets_log2 > 24 || cfg.sample_per_mille < 1 || cfg.sample_per_mille > 1000 ||
+      cfg.points < 1 || !(cfg.lr > 0.f))
+    return fail(c, TS_ERR_INVALID_ARG, "invalid learned-sort configuration");
+  if (report) std::memset(report, 0, sizeof *report);
+  if (n > 0 && !data) return fail(c, TS_ERR_INVALID_ARG, "null data");
+  DeviceScope ds(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  clock_reset(c);
+  if (n <= 1) return TS_OK;
+
+  const uint32_t B = 1u << cfg.buckets_log2;
+  size_t S = n / 1000 * cfg.sample_per_mille + (n % 1000) * cfg.sample_per_mille / 1000;
+  S = std::min(n, std::max<size_t>(S, std::min<size_t>(n, 1024)));
+  const uint32_t T = uint32_t(std::min<size_t>(S, cfg.points));
+  // meta | status | scratch | sample x2 | u | t | theta | grad | loss | model | offsets | seg counts | seg lists
+  // | long-bucket arrays
+  const size_t cap = ts::seg_lds_max_len(kb, false);
+  const size_t fmax = long_keys_max(n, cap);
+  size_t off[15];
+  size_t b = 0;
+  auto take = [&](int i, size_t bytes) { off[i] = b; b += up(bytes); };
+  take(0, sizeof(ts::SortMeta));
+  take(1, ts::status_bytes(n, kb));
+  take(2, n * size_t(kb));
+  take(3, S * 8);
+  take(4, S * 8);
+  take(5, size_t(T) * 4);
+  take(6, size_t(T) * 4);
+  take(7, ts::kKnots * 4);
+  take(8, ts::kKnots * 4 + (ts::kKnots + 1) * 8);  // gradient, then the knots' first sample indices
+  take(9, 4 * 4 + 2 * 8);  // loss at the start / after fitting / of uniform knots, then the largest bucket
+  take(10, sizeof(ts::LearnedModel));
+  take(11, (size_t(B) + 1) * 8);
+  take(12, (ts::kSegClasses + 2) * 4);
+  take(13, size_t(ts::kSegClasses) * B * 4);
+  take(14, long_keys_bytes(fmax));
+  ts_status st = ensure(c, b);
+  if (st != TS_OK) return st;
+  char* w = c->ws;
+  ts::LearnedJob j;
+  j.key_bytes = kb;
+  j.mode = mode;
+  j.data = data;
+  j.scratch = w + off[2];
+  j.n = n;
+  j.sample = S;
+  j.sample_a = reinterpret_cast<unsigned long long*>(w + off[3]);
+  j.sample_b = reinterpret_cast<unsigned long long*>(w + off[4]);
+  j.points = T;
+  j.u = reinterpret_cast<float*>(w + off[5]);
+  j.t = reinterpret_cast<float*>(w + off[6]);
+  j.theta = reinterpret_cast<float*>(w + off[7]);
+  j.grad = reinterpret_cast<float*>(w + off[8]);
+  j.knot_first = reinterpret_cast<unsigned long long*>(w + off[8] + ts::kKnots * 4);
+  j.loss = reinterpret_cast<float*>(w + off[9]);
+  j.stats = reinterpret_cast<unsigned long long*>(w + off[9] + 16);
+  j.train = cfg.train != 0;
+  j.iters = int(cfg.iters);
+  j.lr = cfg.lr;
+  j.model = reinterpret_cast<ts::LearnedModel*>(w + off[10]);
+  j.bucket_shift = 32 - cfg.buckets_log2;
+  j.buckets = B;
+  j.offsets = reinterpret_cast<unsigned long long*>(w + off[11]);
+  j.meta = reinterpret_cast<ts::SortMeta*>(w + off[0]);
+  j.status = w + off[1];
+  j.num_cus = c->num_cus;
+  hipError_t e = ts::learned_partition(j, s, &c->clk);
+  if (e != hipSuccess) return hip_fail(c, e, "learned_partition");
+
+  ts::SegJob sj;
+  sj.key_bytes = kb;
+  sj.mode = mode;
+  sj.keys = data;
+  sj.n = n;
+  sj.offsets = j.offsets;
+  sj.n_segments = B;
+  sj.counts = reinterpret_cast<uint32_t*>(w + off[12]);
+  sj.lists = reinterpret_cast<uint32_t*>(w + off[13]);
+  sj.num_cus = c->num_cus;
+  e = ts::segmented_sort(sj, s, &c->clk);
+  if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
+
+  uint32_t h[ts::kSegClasses + 2];
+  float loss[3] = {0.f, 0.f, 0.f};
+  unsigned long long largest = 0;
+  e = hipMemcpyAsync(h, sj.counts, sizeof h, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && j.train) e = hipMemcpyAsync(loss, j.loss, sizeof loss, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&largest, j.stats, sizeof largest, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(c, e, "learned buckets");
+  const uint32_t nbig = h[ts::kSegDevice];
+  std::vector<std::pair<uint64_t, uint64_t>> fat;
+  uint64_t okeys = 0;
+  if (nbig) {
+    std::vector<uint32_t> ids(nbig);
+    std::vector<unsigned long long> offs(size_t(B) + 1);
+    e = hipMemcpy(ids.data(), sj.lists + size_t(ts::kSegDevice) * B, nbig * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(offs.data(), j.offsets, offs.size() * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "overflow buckets");
+    for (uint32_t i = 0; i < nbig; ++i) {
+      const uint64_t lo = offs[ids[i]], m = offs[ids[i] + 1] - lo;
+      okeys += m;
+      fat.emplace_back(lo, m);
+    }
+  }
+  uint32_t rounds = 0;
+  if (!fat.empty()) {
+    st = sort_long_keys(c, kb, mode, data, j.scratch, n, std::move(fat), w + off[14], fmax, s, &rounds);
+    if (st != TS_OK) return st;
+  }
+  if (report) {
+    report->sample = S;
+    report->buckets = B;
+    report->overflow_buckets = nbig;
+    report->split_rounds = rounds;
+    report->overflow_keys = okeys;
+    report->largest_bucket = largest;
+    report->loss_before = loss[0];
+    report->loss_after = loss[1];
+    report->loss_uniform = loss[2];
+  }
+  return TS_OK;
+}
+
+ts_status ts_pwl_fit(ts_ctx* c, const float* u, const float* t, size_t T, float* theta, float* grad, float* loss2,
+                     int iters, float lr, void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (T == 0 || T > size_t(UINT32_MAX) || !u || !t || !theta || !grad || !loss2 || iters < 0 || !(lr > 0.f))
+    return fail(c, TS_ERR_INVALID_ARG, "invalid fit arguments");
+  DeviceScope ds(c->device);
+  hipError_t e = ts::pwl_fit(u, t, uint32_t(T), theta, grad, loss2, iters, lr, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "pwl_fit");
 }
 
 size_t ts_segmented_lds_max_len(ts_dtype dt, int with_values) {

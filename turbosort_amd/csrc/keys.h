@@ -158,4 +158,50 @@ struct SortMeta {
   unsigned long long check[4];  // checksum scratch: sum, xor, unsorted pairs
 };
 
+// ---------------------------------------------------------------------------
+// Learned partition model: a monotone piecewise-linear CDF over a
+// logarithmic key coordinate, evaluated in integer arithmetic so that the
+// bucket of a key never decreases as the key grows.
+//
+// log2_fixed(k) = e * 2^23 + (the 23 bits below k's leading one), e its
+// position: non-decreasing in k.  The sample's range [xlo, xhi] of that
+// coordinate is stretched onto u in [0, kKnots * 2^19); knot i covers
+// u >> 19 == i, and F(u) = knots[i] + (knots[i+1] - knots[i]) * (u mod 2^19) / 2^19
+// (rounded down) never exceeds knots[i+1], so F is non-decreasing across
+// knots too.  A key's bucket is F >> bucket_shift.
+constexpr int kKnots = 1024;
+constexpr int kKnotFracBits = 19;  // u bits below the knot index
+
+struct LearnedModel {
+  uint32_t knots[kKnots + 1];  // F at each knot, 32-bit fixed point, non-decreasing
+  uint32_t bucket_shift;       // 32 - log2(number of buckets)
+  uint32_t pad;
+  unsigned long long xlo, xhi;  // coordinate range of the sample
+  unsigned long long mul;       // floor(2^61 / (xhi - xlo + 1))
+};
+
+__host__ __device__ __forceinline__ unsigned long long log2_fixed(unsigned long long k) {
+  if (k == 0) return 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const int e = 63 - __clzll((long long)k);
+#else
+  const int e = 63 - __builtin_clzll(k);
+#endif
+  const unsigned long long frac = e >= 23 ? (k >> (e - 23)) : (k << (23 - e));
+  return (unsigned long long)e << 23 | (frac & 0x7FFFFFull);
+}
+
+// The model coordinate of a radix key: u in [0, kKnots << kKnotFracBits).
+__host__ __device__ __forceinline__ uint32_t learned_coord(unsigned long long key, const LearnedModel& m) {
+  unsigned long long x = log2_fixed(key);
+  x = x < m.xlo ? m.xlo : (x > m.xhi ? m.xhi : x);
+  return uint32_t(((x - m.xlo) * m.mul) >> 32);
+}
+
+__host__ __device__ __forceinline__ uint32_t learned_cdf(uint32_t u, const uint32_t* knots) {
+  const uint32_t i = u >> kKnotFracBits, f = u & ((1u << kKnotFracBits) - 1);
+  const uint32_t y0 = knots[i], y1 = knots[i + 1];
+  return y0 + uint32_t((unsigned long long)(y1 - y0) * f >> kKnotFracBits);
+}
+
 }  // namespace ts

@@ -94,6 +94,10 @@ struct RadixJob {
   SortMeta* meta = nullptr;
   void* status = nullptr;
   int num_cus = 256;
+  // Learned partition: the passes sort by the bytes of each key's model
+  // bucket (up to 24 bits) instead of by the key (keys only; 4- and 8-byte
+  // keys).
+  const LearnedModel* model = nullptr;
 };
 
 hipError_t radix_sort(const RadixJob& job, hipStream_t s, KernelClock* clk);
@@ -123,14 +127,75 @@ struct SegJob {
   void* keys = nullptr;
   uint32_t* vals = nullptr;  // null: keys only
   size_t n = 0;
-  const unsigned long long* offsets = nullptr;  // n_segments + 1 entries
+  const unsigned long long* offsets = nullptr;  // n_segments + 1 entries (segment starts)
+  const unsigned long long* ends = nullptr;     // optional: segment s is [offsets[s], ends[s])
   size_t n_segments = 0;
   uint32_t* counts = nullptr;  // kSegClasses + 2 words
   uint32_t* lists = nullptr;   // kSegClasses * n_segments words
   int num_cus = 256;
 };
 uint32_t seg_lds_max_len(int key_bytes, bool vals);
+
+// Three-way split of long keys-only segments (segmented.hip): segment s is
+// [lo[s], lo[s] + len[s]), cut into chunks of split_chunk() keys numbered
+// chunk_start[s] .. chunk_start[s + 1] - 1 (chunks in total).  Keys below a
+// pivot (a median of five) move to the front, keys above it to the back, the
+// middle is filled with the pivot; below[s] and above[s] receive the counts.
+struct SplitJob {
+  int key_bytes = 8;
+  int mode = kIdentity;
+  void* data = nullptr;
+  void* tmp = nullptr;  // as long as data
+  uint32_t nseg = 0;
+  uint32_t chunks = 0;
+  const unsigned long long* lo = nullptr;
+  const unsigned long long* len = nullptr;
+  const uint32_t* chunk_start = nullptr;  // nseg entries
+  unsigned long long* below = nullptr;
+  unsigned long long* above = nullptr;
+  unsigned long long* pivot = nullptr;
+};
+uint32_t split_chunk();
+hipError_t split_segments(const SplitJob& job, hipStream_t s);
 hipError_t segmented_sort(const SegJob& job, hipStream_t s, KernelClock* clk);
+
+// Learned partition (learned.hip): sample, fit, then group the keys of `data`
+// by model bucket in place and write the bucket bounds.  Every buffer is
+// caller-provided device memory.
+struct LearnedJob {
+  int key_bytes = 8;
+  int mode = kIdentity;
+  void* data = nullptr;
+  void* scratch = nullptr;  // n keys
+  size_t n = 0;
+  size_t sample = 0;
+  unsigned long long* sample_a = nullptr;  // sample keys, two halves
+  unsigned long long* sample_b = nullptr;
+  uint32_t points = 0;  // training points
+  float* u = nullptr;
+  float* t = nullptr;
+  float* theta = nullptr;  // kKnots
+  float* grad = nullptr;   // kKnots
+  float* loss = nullptr;   // 3: at the start, after fitting, of uniform knots
+  unsigned long long* knot_first = nullptr;  // kKnots + 1
+  bool train = true;
+  int iters = 300;
+  float lr = 0.05f;
+  LearnedModel* model = nullptr;
+  uint32_t bucket_shift = 16;
+  uint32_t buckets = 65536;
+  unsigned long long* offsets = nullptr;  // buckets + 1
+  unsigned long long* stats = nullptr;    // [0]: keys in the largest bucket
+  SortMeta* meta = nullptr;
+  void* status = nullptr;  // status_bytes(n, key_bytes)
+  int num_cus = 256;
+};
+hipError_t learned_partition(const LearnedJob& job, hipStream_t s, KernelClock* clk);
+
+// One fit of the piecewise-linear CDF (see k_pwl_fit): theta in/out, the
+// gradient at the initial theta, the loss before and after `iters` steps.
+hipError_t pwl_fit(const float* u, const float* t, uint32_t T, float* theta, float* grad, float* loss, int iters,
+                   float lr, hipStream_t s);
 
 hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStream_t s);
 hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, SortMeta* meta, hipStream_t s);

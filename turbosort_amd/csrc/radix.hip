@@ -76,16 +76,41 @@ template <class U> struct HistUnroll {
   static constexpr int value = sizeof(U) >= 8 ? 8 : 16;
 };
 
-template <class U, bool REC>
+// The digits of a pass come from the radix key itself, or (LEARNED) from the
+// bucket (at most 24 bits) the learned model gives the key: three byte
+// digits, of which the scan skips those no bucket id uses.
+template <class U, bool LEARNED> struct DigitSource {
+  using T = U;
+  static constexpr int P = int(sizeof(U));
+};
+template <class U> struct DigitSource<U, true> {
+  using T = uint32_t;
+  static constexpr int P = 3;
+};
+
+// Copies the model's knots into LDS (every lane calls; a barrier follows).
+__device__ __forceinline__ void load_knots(uint32_t* lds, const LearnedModel* m) {
+  for (unsigned i = threadIdx.x; i <= unsigned(kKnots); i += blockDim.x) lds[i] = m->knots[i];
+}
+
+template <class U>
+__device__ __forceinline__ uint32_t bucket_of(U k, const uint32_t* knots, const LearnedModel& m, uint32_t shift) {
+  return learned_cdf(learned_coord((unsigned long long)k, m), knots) >> shift;
+}
+
+template <class U, bool REC, bool LEARNED = false>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ keys, const unsigned char* __restrict__ rec,
                                                        size_t stride, size_t key_off, U* __restrict__ keys_out,
                                                        size_t n, int mode, size_t seg0, uint32_t blocks_per_seg,
                                                        size_t per_block, SortMeta* m, uint32_t* zero_ptr,
-                                                       size_t zero_words) {
-  constexpr int P = int(sizeof(U));
+                                                       size_t zero_words, const LearnedModel* model) {
+  using D = typename DigitSource<U, LEARNED>::T;
+  constexpr int P = DigitSource<U, LEARNED>::P;
   constexpr int UN = HistUnroll<U>::value;
   __shared__ uint32_t h[P * kSegments * 256];
+  __shared__ uint32_t knots[LEARNED ? kKnots + 1 : 1];
   const unsigned tid = threadIdx.x, lane = tid & 63;
+  if (LEARNED) load_knots(knots, model);
   for (unsigned i = tid; i < P * kSegments * 256; i += kHistThreads) h[i] = 0;
   {
     const size_t g = size_t(blockIdx.x) * kHistThreads + tid, gs = size_t(gridDim.x) * kHistThreads;
@@ -113,14 +138,22 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ key
         if (full || i < end) keys_out[i] = k[r];
       }
     }
-    // Bits in which the wave's valid keys differ from its first key.
-    const U k0 = first_lane(k[0]);
+    D dk[UN];
+#pragma unroll
+    for (int r = 0; r < UN; ++r) {
+      if constexpr (LEARNED)
+        dk[r] = bucket_of(k[r], knots, *model, model->bucket_shift);
+      else
+        dk[r] = k[r];
+    }
+    // Bits in which the wave's valid digit sources differ from its first.
+    const D k0 = first_lane(dk[0]);
     uint64_t diff = 0;
     uint32_t wave_count = 64u * UN;
 #pragma unroll
     for (int r = 0; r < UN; ++r) {
       const size_t i = base + size_t(r) * kHistThreads + tid;
-      if (full || i < end) diff |= uint64_t(k[r] ^ k0);
+      if (full || i < end) diff |= uint64_t(dk[r] ^ k0);
     }
     if (!full) {
       wave_count = 0;
@@ -141,8 +174,8 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ key
         for (int r = 0; r < UN; ++r) {
           const size_t i = base + size_t(r) * kHistThreads + tid;
           if (full || i < end) {
-            const uint32_t seg = p == 0 ? s0 : uint32_t(uint64_t(k[r]) >> (8 * p - 3)) & 7u;
-            atomicAdd(&hp[seg * 256 + (uint32_t(uint64_t(k[r]) >> (8 * p)) & 0xFFu)], 1u);
+            const uint32_t seg = p == 0 ? s0 : uint32_t(uint64_t(dk[r]) >> (8 * p - 3)) & 7u;
+            atomicAdd(&hp[seg * 256 + (uint32_t(uint64_t(dk[r]) >> (8 * p)) & 0xFFu)], 1u);
           }
         }
       }
@@ -254,6 +287,7 @@ template <class U> struct SweepArgs {
   uint32_t* tickets;
   uint32_t* error;  // set when a look-back wait exceeds its bound
   uint32_t num_tiles;
+  const LearnedModel* model;  // learned partition: digits of the key's bucket
 };
 
 // About two seconds of s_sleep(1) polling.
@@ -345,12 +379,14 @@ __device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[(
 // pass applies the inverse key transform, and in a by-key sort writes only the
 // order to final_vals.
 // LDS of one scatter tile, declared once in the kernel.
-template <class U, bool VALS, int IPT> struct SweepLds {
+template <class U, bool VALS, int IPT, bool LEARNED = false> struct SweepLds {
   static constexpr int NW = SweepShape<sizeof(U)>::threads / 64;
   static constexpr int TILE = SweepShape<sizeof(U)>::threads * IPT;
   uint32_t whist[NW][256];  // per-wave digit counts, then staging bases
   U keys[TILE];
   uint32_t vals[VALS ? TILE : 1];
+  uint8_t sdig[LEARNED ? TILE : 1];  // staged keys' digits (learned digits are not a key field)
+  uint32_t knots[LEARNED ? kKnots + 1 : 1];
   unsigned long long gofs[256];
   uint32_t wave[NW];
   uint32_t tcnt[256];
@@ -360,8 +396,8 @@ template <class U, bool VALS, int IPT> struct SweepLds {
 // Everything a tile does after it knows where it is: load, publish its digit
 // counts, rank, look back, stage, store.  FULL tiles (all but each segment's
 // last) run without any per-key bounds test.
-template <class U, bool VALS, class S, bool FULL>
-__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>::ipt>& L, const SweepArgs<U>& a,
+template <class U, bool VALS, class S, bool FULL, bool LEARNED>
+__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>::ipt, LEARNED>& L, const SweepArgs<U>& a,
                                            S* __restrict__ status, int pass, unsigned shift, bool first, bool last,
                                            size_t tile_base, uint32_t valid, size_t first_slot, size_t slot,
                                            bool first_round, unsigned long long digit_base) {
@@ -405,13 +441,24 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>
     }
     if (first) to_keys(k, a.mode_in);
   }
+  uint32_t bk[LEARNED ? IPT : 1];
+  if constexpr (LEARNED) {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) bk[j] = bucket_of(k[j], L.knots, *a.model, a.model->bucket_shift);
+  }
+  auto dig = [&](int j) {
+    if constexpr (LEARNED)
+      return digit_of(bk[j], shift);
+    else
+      return digit_of(k[j], shift);
+  };
 
   // Publish the tile's digit counts before ranking, so that later tiles
   // rarely find this one unpublished.
   S* my = row + slot * 256 + tid;
 #pragma unroll
   for (int j = 0; j < IPT; ++j)
-    if (in_tile(j)) atomicAdd(&L.tcnt[digit_of(k[j], shift)], 1u);
+    if (in_tile(j)) atomicAdd(&L.tcnt[dig(j)], 1u);
   __syncthreads();
   if (dl) __hip_atomic_store(my, S(L.tcnt[tid]) | (first_round ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // Emit nothing, but make the keys opaque so that the digits computed above
@@ -420,7 +467,10 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>
   for (int j = 0; j < IPT; ++j) opaque(k[j]);
 
   uint32_t rk[(IPT + 1) / 2];
-  rank_in_wave<U, IPT, FULL>(k, rk, shift, L.whist[w], lb, valid);
+  if constexpr (LEARNED)
+    rank_in_wave<uint32_t, IPT, FULL>(bk, rk, shift, L.whist[w], lb, valid);
+  else
+    rank_in_wave<U, IPT, FULL>(k, rk, shift, L.whist[w], lb, valid);
 #pragma unroll
   for (int j = 0; j < IPT; ++j) opaque(k[j]);  // again: recompute digits when staging
   __syncthreads();
@@ -445,10 +495,11 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     if (in_tile(j)) {
-      const uint32_t d = digit_of(k[j], shift);
+      const uint32_t d = dig(j);
       const uint32_t pos = L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
       L.keys[pos] = k[j];
       if (VALS) L.vals[pos] = v[j];
+      if (LEARNED) L.sdig[pos] = uint8_t(d);
     }
     // Keep the scheduler from hoisting every iteration's LDS read ahead of
     // the writes, which costs more registers than it hides latency.
@@ -510,7 +561,8 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>
       const uint32_t s = uint32_t(j) * THREADS + tid;
       if (FULL || s < valid) {
         const U kk = L.keys[s];
-        const size_t o = size_t(L.gofs[digit_of(kk, shift)]) + s;
+        const uint32_t d = LEARNED ? uint32_t(L.sdig[s]) : digit_of(kk, shift);
+        const size_t o = size_t(L.gofs[d]) + s;
         if (keys_out) dst[o] = transform(kk);
         if (VALS) vdst[o] = L.vals[s];
       }
@@ -546,13 +598,13 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>
 // Pass 0 of a by-key sort generates the values (positions 0..n-1); the last
 // pass applies the inverse key transform, and in a by-key sort writes only the
 // order to final_vals.
-template <class U, bool VALS, class S>
+template <class U, bool VALS, class S, bool LEARNED = false>
 __global__ __launch_bounds__(SweepShape<sizeof(U)>::threads) void k_sweep(SweepArgs<U> a, S* __restrict__ status, int pass) {
   constexpr int IPT = SweepShape<sizeof(U)>::ipt;
   constexpr int THREADS = SweepShape<sizeof(U)>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
-  __shared__ SweepLds<U, VALS, IPT> L;
+  __shared__ SweepLds<U, VALS, IPT, LEARNED> L;
 
   const SortMeta* __restrict__ meta = a.meta;
   const uint32_t n_live = meta->n_live;
@@ -573,6 +625,7 @@ __global__ __launch_bounds__(SweepShape<sizeof(U)>::threads) void k_sweep(SweepA
 
   for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
   if (tid < 256) L.tcnt[tid] = 0;
+  if (LEARNED) load_knots(L.knots, a.model);
   __syncthreads();
   const uint32_t ticket = __builtin_amdgcn_readfirstlane(L.ticket);
   S* next_row = status + size_t((pass + 1) & 1) * a.num_tiles * 256;
@@ -593,11 +646,11 @@ __global__ __launch_bounds__(SweepShape<sizeof(U)>::threads) void k_sweep(SweepA
   const unsigned long long digit_base =
       tid < 256 ? meta->base[digit][tid] + meta->segpre[digit][seg][tid] : 0ull;
   if (valid == uint32_t(TILE))
-    run_tile<U, VALS, S, true>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot, slot,
-                                 round == 0, digit_base);
+    run_tile<U, VALS, S, true, LEARNED>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
+                                          slot, round == 0, digit_base);
   else
-    run_tile<U, VALS, S, false>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot, slot,
-                                  round == 0, digit_base);
+    run_tile<U, VALS, S, false, LEARNED>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
+                                           slot, round == 0, digit_base);
 }
 
 // An odd number of live passes leaves the keys in the scratch half.
@@ -615,7 +668,7 @@ __global__ void k_iota_if_dead(const SortMeta* m, uint32_t* out, size_t n) {
   for (size_t i = g; i < n; i += gs) out[i] = uint32_t(i);
 }
 
-template <class U, class S> hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk) {
+template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk) {
   const uint32_t tiles = sweep_tiles(j.n, sizeof(U));
   SweepArgs<U> a;
   a.keys_a = static_cast<U*>(j.keys_a);
@@ -631,11 +684,15 @@ template <class U, class S> hipError_t run_sweeps(const RadixJob& j, hipStream_t
   a.tickets = j.meta->tickets;
   a.error = &j.meta->error;
   a.num_tiles = tiles;
+  a.model = j.model;
   const bool vals = j.final_vals != nullptr;
   const double bytes = double(j.n) * (2.0 * sizeof(U) + (vals ? 8.0 : 0.0));
-  for (int p = 0; p < int(sizeof(U)); ++p) {
+  for (int p = 0; p < DigitSource<U, LEARNED>::P; ++p) {
     if (clk) clk->begin(s, "k_sweep", bytes);
-    if (vals)
+    if constexpr (LEARNED)
+      hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a,
+                         static_cast<S*>(j.status), p);
+    else if (vals)
       hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
     else
       hipLaunchKernelGGL((k_sweep<U, false, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
@@ -644,31 +701,37 @@ template <class U, class S> hipError_t run_sweeps(const RadixJob& j, hipStream_t
   return hipGetLastError();
 }
 
-template <class U> hipError_t run_radix(const RadixJob& j, hipStream_t s, KernelClock* clk) {
-  constexpr int P = int(sizeof(U));
+template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j, hipStream_t s, KernelClock* clk) {
+  constexpr int P = DigitSource<U, LEARNED>::P;
+  constexpr int KB = int(sizeof(U));
   hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, s, j.meta, P);
   // Histogram: blocks_per_seg workgroups per position segment, each over a
   // contiguous chunk of it.
-  const size_t seg0 = first_segment(j.n, P);
+  const size_t seg0 = first_segment(j.n, KB);
   const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * 4 / kSegments));
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
   size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
   per_block = std::max(step, ((per_block + step - 1) / step) * step);
-  const uint32_t tiles = sweep_tiles(j.n, P);
+  const uint32_t tiles = sweep_tiles(j.n, KB);
   const size_t zero_words = size_t(tiles) * 256 * (wide_status(j.n) ? 2 : 1);
   const dim3 hgrid(blocks_per_seg * kSegments);
-  if (clk) clk->begin(s, "k_hist", double(j.n) * (j.records ? double(j.stride) + P : P));
-  if (j.records)
+  if (clk) clk->begin(s, "k_hist", double(j.n) * (j.records ? double(j.stride) + KB : KB));
+  if constexpr (LEARNED)
+    hipLaunchKernelGGL((k_hist<U, false, true>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
+                       nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
+                       static_cast<uint32_t*>(j.status), zero_words, j.model);
+  else if (j.records)
     hipLaunchKernelGGL((k_hist<U, true>), hgrid, dim3(kHistThreads), 0, s, nullptr, j.records, j.stride,
                        j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                       static_cast<uint32_t*>(j.status), zero_words);
+                       static_cast<uint32_t*>(j.status), zero_words, nullptr);
   else
     hipLaunchKernelGGL((k_hist<U, false>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a), nullptr,
                        size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                       static_cast<uint32_t*>(j.status), zero_words);
+                       static_cast<uint32_t*>(j.status), zero_words, nullptr);
   if (clk) clk->end(s);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(P)));
-  hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t>(j, s, clk) : run_sweeps<U, uint32_t>(j, s, clk);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(KB)));
+  hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk)
+                                  : run_sweeps<U, uint32_t, LEARNED>(j, s, clk);
   if (e != hipSuccess) return e;
   const unsigned cb = unsigned(std::min<size_t>((j.n + 255) / 256, size_t(j.num_cus) * 8));
   if (j.final_vals) {
@@ -683,6 +746,14 @@ template <class U> hipError_t run_radix(const RadixJob& j, hipStream_t s, Kernel
 }  // namespace
 
 hipError_t radix_sort(const RadixJob& job, hipStream_t s, KernelClock* clk) {
+  if (job.model) {
+    if (job.records || job.final_vals) return hipErrorInvalidValue;
+    switch (job.key_bytes) {
+      case 4: return run_radix<uint32_t, true>(job, s, clk);
+      case 8: return run_radix<uint64_t, true>(job, s, clk);
+      default: return hipErrorInvalidValue;
+    }
+  }
   switch (job.key_bytes) {
     case 1: return run_radix<uint8_t>(job, s, clk);
     case 2: return run_radix<uint16_t>(job, s, clk);

@@ -33,6 +33,7 @@ struct SegArgs {
   void* keys;
   uint32_t* vals;  // null: keys only
   const unsigned long long* offsets;
+  const unsigned long long* ends;  // null: segment s ends at offsets[s + 1]
   size_t n;
   size_t n_segments;
   uint32_t* counts;  // per class, then the invalid-offsets flag
@@ -40,6 +41,10 @@ struct SegArgs {
   uint32_t caps[kSegClasses];
   int mode;
 };
+
+__device__ __forceinline__ unsigned long long seg_end(const SegArgs& a, size_t s) {
+  return a.ends ? a.ends[s] : a.offsets[s + 1];
+}
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
@@ -77,7 +82,7 @@ __global__ __launch_bounds__(kClassifyThreads) void k_seg_classify(SegArgs a) {
     const size_t s = chunk + size_t(r) * kClassifyThreads + tid;
     int c = -1;
     if (s < a.n_segments) {
-      const unsigned long long lo = a.offsets[s], hi = a.offsets[s + 1];
+      const unsigned long long lo = a.offsets[s], hi = seg_end(a, s);
       if (hi < lo || hi > a.n) {
         a.counts[kSegClasses] = 1;
       } else if (hi - lo >= 2) {
@@ -135,7 +140,7 @@ template <class U, int W, bool VALS> __global__ __launch_bounds__(256) void k_se
     if (job < count) {
       const uint32_t seg = list[job];
       lo = a.offsets[seg];
-      len = uint32_t(a.offsets[seg + 1] - lo);
+      len = uint32_t(seg_end(a, seg) - lo);
     }
     C k = C(U(~U(0)));
     uint32_t v = 0;
@@ -211,7 +216,7 @@ template <class U, int T, int IPT, bool VALS> __global__ __launch_bounds__(T) vo
   for (uint32_t job = blockIdx.x; job < count; job += gridDim.x) {
     const uint32_t seg = list[job];
     const size_t lo = a.offsets[seg];
-    const uint32_t len = uint32_t(a.offsets[seg + 1] - lo);
+    const uint32_t len = uint32_t(seg_end(a, seg) - lo);
     U* __restrict__ keys = static_cast<U*>(a.keys) + lo;
     uint32_t* __restrict__ vals = VALS ? a.vals + lo : nullptr;
 
@@ -397,6 +402,156 @@ template <class U, int T, int IPT, bool VALS> __global__ __launch_bounds__(T) vo
 }
 
 // ---------------------------------------------------------------------------
+// Three-way split of long keys-only segments around a pivot key (equal keys
+// are equal bits, so no order among them is observable): keys below the
+// pivot go to the front of the segment, keys above to the back, and the
+// middle becomes copies of the pivot.  Split segment s is
+// [lo[s], lo[s] + len[s]); its chunks of kSplitChunk keys are
+// chunk_start[s] .. chunk_start[s + 1] - 1.
+
+constexpr int kSplitThreads = 256;
+constexpr int kSplitIpt = 16;
+constexpr uint32_t kSplitChunk = uint32_t(kSplitThreads) * kSplitIpt;
+
+struct SplitArgs {
+  void* data;
+  void* tmp;
+  int mode;
+  uint32_t nseg;
+  const unsigned long long* lo;
+  const unsigned long long* len;
+  const uint32_t* chunk_start;
+  unsigned long long* below;  // keys below the pivot, per segment
+  unsigned long long* above;
+  unsigned long long* pivot;  // radix key
+};
+
+// Pivot: the median of five keys spread over the segment; counters zeroed.
+template <class U> __global__ void k_split_pivot(SplitArgs a) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.nseg) return;
+  const U* d = static_cast<const U*>(a.data) + a.lo[s];
+  const unsigned long long len = a.len[s];
+  U v[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) v[k] = to_key<U>(d[len * (2 * k + 1) / 10], a.mode);
+#pragma unroll
+  for (int x = 0; x < 5; ++x)
+#pragma unroll
+    for (int y = 0; y + 1 < 5 - x; ++y)
+      if (v[y + 1] < v[y]) {
+        const U t = v[y];
+        v[y] = v[y + 1];
+        v[y + 1] = t;
+      }
+  a.pivot[s] = (unsigned long long)v[2];
+  a.below[s] = 0;
+  a.above[s] = 0;
+}
+
+__device__ __forceinline__ uint32_t split_segment(const SplitArgs& a, uint32_t chunk) {
+  uint32_t lo = 0, hi = a.nseg - 1;  // largest s with chunk_start[s] <= chunk
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (a.chunk_start[mid] <= chunk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class U> __global__ __launch_bounds__(kSplitThreads) void k_split(SplitArgs a) {
+  __shared__ uint32_t wtot[kSplitThreads / 64][2];
+  __shared__ unsigned long long base[2];
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t s = split_segment(a, blockIdx.x);
+  const unsigned long long slo = a.lo[s], slen = a.len[s];
+  const unsigned long long lo = slo + (unsigned long long)(blockIdx.x - a.chunk_start[s]) * kSplitChunk;
+  const unsigned long long hi = min(slo + slen, lo + kSplitChunk);
+  const U p = U(a.pivot[s]);
+  const U* __restrict__ d = static_cast<const U*>(a.data);
+  U* __restrict__ tmp = static_cast<U*>(a.tmp);
+  U x[kSplitIpt];
+#pragma unroll
+  for (int j = 0; j < kSplitIpt; ++j) x[j] = d[min(lo + (unsigned long long)j * kSplitThreads + tid, hi - 1)];
+  // Ranks among this wave's keys of the same side, in key order.
+  uint32_t rank[kSplitIpt];
+  uint32_t nb = 0, na = 0;
+#pragma unroll
+  for (int j = 0; j < kSplitIpt; ++j) {
+    const bool valid = lo + (unsigned long long)j * kSplitThreads + tid < hi;
+    const U k = to_key<U>(x[j], a.mode);
+    const bool b = valid && k < p, ab = valid && p < k;
+    const uint64_t mb = __ballot(b), ma = __ballot(ab);
+    rank[j] = b ? nb + lanes_below(mb) : (ab ? na + lanes_below(ma) : 0xFFFFFFFFu);
+    if (ab) rank[j] |= 0x80000000u;
+    nb += uint32_t(__popcll(mb));
+    na += uint32_t(__popcll(ma));
+  }
+  if (lane == 0) {
+    wtot[w][0] = nb;
+    wtot[w][1] = na;
+  }
+  __syncthreads();
+  uint32_t ob = 0, oa = 0, tb = 0, ta = 0;
+#pragma unroll
+  for (int x2 = 0; x2 < kSplitThreads / 64; ++x2) {
+    ob += x2 < int(w) ? wtot[x2][0] : 0u;
+    oa += x2 < int(w) ? wtot[x2][1] : 0u;
+    tb += wtot[x2][0];
+    ta += wtot[x2][1];
+  }
+  if (tid == 0) {
+    base[0] = tb ? atomicAdd(&a.below[s], (unsigned long long)tb) : 0ull;
+    base[1] = ta ? atomicAdd(&a.above[s], (unsigned long long)ta) : 0ull;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSplitIpt; ++j) {
+    const uint32_t r = rank[j];
+    if (r == 0xFFFFFFFFu) continue;
+    if (r & 0x80000000u)
+      tmp[slo + slen - 1 - (base[1] + oa + (r & 0x7FFFFFFFu))] = x[j];
+    else
+      tmp[slo + base[0] + ob + r] = x[j];
+  }
+}
+
+template <class U> __global__ __launch_bounds__(kSplitThreads) void k_split_fill(SplitArgs a) {
+  const uint32_t s = split_segment(a, blockIdx.x);
+  const unsigned long long slo = a.lo[s], slen = a.len[s];
+  const unsigned long long lo = slo + (unsigned long long)(blockIdx.x - a.chunk_start[s]) * kSplitChunk;
+  const unsigned long long hi = min(slo + slen, lo + kSplitChunk);
+  const unsigned long long nb = a.below[s], na = a.above[s];
+  const U pv = from_key<U>(U(a.pivot[s]), a.mode);
+  U* __restrict__ d = static_cast<U*>(a.data);
+  const U* __restrict__ tmp = static_cast<const U*>(a.tmp);
+  for (unsigned long long i = lo + threadIdx.x; i < hi; i += kSplitThreads) {
+    const unsigned long long o = i - slo;
+    d[i] = (o < nb || o >= slen - na) ? tmp[i] : pv;
+  }
+}
+
+template <class U> hipError_t run_split(const SplitJob& j, hipStream_t s) {
+  SplitArgs a;
+  a.data = j.data;
+  a.tmp = j.tmp;
+  a.mode = j.mode;
+  a.nseg = j.nseg;
+  a.lo = j.lo;
+  a.len = j.len;
+  a.chunk_start = j.chunk_start;
+  a.below = j.below;
+  a.above = j.above;
+  a.pivot = j.pivot;
+  hipLaunchKernelGGL((k_split_pivot<U>), dim3((j.nseg + 255) / 256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_split<U>), dim3(j.chunks), dim3(kSplitThreads), 0, s, a);
+  hipLaunchKernelGGL((k_split_fill<U>), dim3(j.chunks), dim3(kSplitThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Class table.  Class c takes lengths in (cap[c-1], cap[c]]; a cap of 0 marks
 // a class this key width does not use.
 
@@ -451,6 +606,7 @@ template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStrea
   a.keys = j.keys;
   a.vals = VALS ? j.vals : nullptr;
   a.offsets = j.offsets;
+  a.ends = j.ends;
   a.n = j.n;
   a.n_segments = j.n_segments;
   a.counts = j.counts;
@@ -491,6 +647,19 @@ uint32_t seg_lds_max_len(int key_bytes, bool vals) {
     case 4: return vals ? max_cap<uint32_t, true>() : max_cap<uint32_t, false>();
     case 8: return vals ? max_cap<uint64_t, true>() : max_cap<uint64_t, false>();
     default: return 0;
+  }
+}
+
+uint32_t split_chunk() { return kSplitChunk; }
+
+hipError_t split_segments(const SplitJob& j, hipStream_t s) {
+  if (j.nseg == 0) return hipSuccess;
+  switch (j.key_bytes) {
+    case 1: return run_split<uint8_t>(j, s);
+    case 2: return run_split<uint16_t>(j, s);
+    case 4: return run_split<uint32_t>(j, s);
+    case 8: return run_split<uint64_t>(j, s);
+    default: return hipErrorInvalidValue;
   }
 }
 
