@@ -38,8 +38,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="u32", choices=sorted(WORKLOADS))
     p.add_argument("--log2n", type=int, default=28)
-    p.add_argument("--also", default="u64,u64z,segmented,by_key",
-                   help="extra workloads reported beside the headline (u32, u64, u64z, segmented, by_key)")
+    p.add_argument("--also", default="u64,u64z,segmented,by_key,learned",
+                   help="extra workloads reported beside the headline (u32, u64, u64z, segmented, by_key, learned)")
     p.add_argument("--cpu-seconds", type=float, default=8.0, help="wall budget of the CPU baseline")
     p.add_argument("--no-cpu-baseline", action="store_true")
     return p.parse_args()
@@ -278,6 +278,116 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5):
                 apply_order_ms=dt_apply * 1e3, apply_order_GBps=20 * n / dt_apply / 1e9, ok=ok)
 
 
+GAMMA = 0x9E3779B97F4A7C15
+
+
+def dev_unit(n: int, seed: int, start: int, ctx):
+    """Uniform doubles in [0, 1) from SplitMix64 outputs start .. start+n-1 of
+    the stream with state seed (generated on the device)."""
+    import torch
+
+    import turbosort_amd as ts
+
+    bits = torch.empty(n, dtype=torch.uint64, device="cuda")
+    ts.fill_splitmix64(bits, (seed + start * GAMMA) % (1 << 64), ctx=ctx)
+    b = bits.view(torch.int64)
+    return ((b >> 11) & ((1 << 53) - 1)).to(torch.float64) * (1.0 / 9007199254740992.0)
+
+
+def lognormal_keys_dev(n: int, seed: int, ctx, as_f64: bool = False, hot_share: float = 0.05, n_hot: int = 10_000,
+                       chunk: int = 1 << 27):
+    """The learned-partitioner workload on the device: a mixture of lognormals
+    (mu 0, sigma 0.5 / 1 / 2, weights 0.5 / 0.3 / 0.2) with 5 % of the
+    positions replaced by Zipf(1.1) draws over 10^4 hot values from the same
+    mixture; u64 key = floor(min(x / e^12, 1 - 2^-53) * 2^64), or x as f64.
+    Streams: SplitMix64 states seed + k * 2^32 (the formulas of the oracle's
+    generator, evaluated in float64 on the device)."""
+    import math
+
+    import torch
+
+    st = lambda k: seed + k * (1 << 32)  # noqa: E731
+
+    def mixture(m, start, k0):
+        u1 = dev_unit(m, st(k0), start, ctx)
+        u2 = dev_unit(m, st(k0 + 1), start, ctx)
+        pick = dev_unit(m, st(k0 + 2), start, ctx)
+        z = torch.sqrt(-2.0 * torch.log1p(-u1)) * torch.cos(2.0 * math.pi * u2)
+        sigma = torch.where(pick < 0.5, 0.5, torch.where(pick < 0.8, 1.0, 2.0))
+        return torch.exp(sigma * z)
+
+    hot = mixture(n_hot, 0, 3)
+    w = torch.arange(1, n_hot + 1, dtype=torch.float64, device="cuda") ** -1.1
+    cdf = torch.cumsum(w, 0) / w.sum()
+    out = torch.empty(n, dtype=torch.float64 if as_f64 else torch.uint64, device="cuda")
+    for c0 in range(0, n, chunk):
+        m = min(chunk, n - c0)
+        x = mixture(m, c0, 0)
+        rank = torch.clamp(torch.searchsorted(cdf, dev_unit(m, st(6), c0, ctx), right=True), max=n_hot - 1)
+        x = torch.where(dev_unit(m, st(7), c0, ctx) < hot_share, hot[rank], x)
+        if as_f64:
+            out[c0:c0 + m] = x
+        else:
+            v = torch.clamp(x / math.exp(12.0), max=1.0 - 2.0 ** -53)
+            hi = torch.floor(v * 4294967296.0)
+            lo = torch.floor((v * 4294967296.0 - hi) * 4294967296.0)
+            out.view(torch.int64)[c0:c0 + m] = (hi.to(torch.int64) << 32) | lo.to(torch.int64)
+        del x
+    return out
+
+
+def equal_chunked(a, b, chunk: int = 1 << 26) -> bool:
+    import torch
+
+    a, b = a.view(torch.int64) if a.element_size() == 8 else a, b.view(torch.int64) if b.element_size() == 8 else b
+    return all(bool(torch.equal(a[i:i + chunk], b[i:i + chunk])) for i in range(0, a.numel(), chunk))
+
+
+def run_learned(steps: int, ctx, n: int = 1_000_000_000, seed: int = 8):
+    """Learned partitioner against the plain device radix on 10^9 skewed u64
+    keys (and the same x as f64), with the untrained model as the ablation."""
+    import torch
+
+    import turbosort_amd as ts
+
+    res = {}
+    for name, as_f64 in (("u64", False), ("f64", True)):
+        src = lognormal_keys_dev(n, seed, ctx, as_f64=as_f64)
+        torch.cuda.synchronize()
+        ctx.reserve(src.dtype, n)
+        warm = 1
+
+        def timed_copies(fn):
+            bufs = [src.clone() for _ in range(steps + warm)]
+            for i in range(warm):
+                fn(bufs[i])
+            dt = timed(lambda i: fn(bufs[warm + i]), steps)
+            out = bufs[-1]
+            del bufs[:-1]
+            return dt, out
+
+        dt_plain, ref = timed_copies(lambda b: ts.sort(b, ctx=ctx))
+        rep = {}
+        dt_learn, out = timed_copies(lambda b: rep.update(ts.learned_sort(b, ctx=ctx)))
+        ok = equal_chunked(out, ref)
+        del out
+        rep_u = {}
+        dt_untr, out = timed_copies(lambda b: rep_u.update(ts.learned_sort(b, config=ts.LearnedConfig(train=False),
+                                                                            ctx=ctx)))
+        ok = ok and equal_chunked(out, ref)
+        del out, ref, src
+        torch.cuda.empty_cache()
+        res[name] = dict(n=n, plain_ms=dt_plain * 1e3, learned_ms=dt_learn * 1e3, untrained_ms=dt_untr * 1e3,
+                         plain_keys_per_s=n / dt_plain, learned_keys_per_s=n / dt_learn,
+                         speedup_vs_plain=dt_plain / dt_learn, untrained_speedup_vs_plain=dt_plain / dt_untr,
+                         learned_roofline_frac=40 * n / dt_learn / 1e9 / HBM_PEAK_GBS,
+                         plain_roofline_frac=136 * n / dt_plain / 1e9 / HBM_PEAK_GBS,
+                         overflow_rate=rep["overflow_keys"] / n, report=rep, untrained_report=rep_u, ok=ok)
+    return dict(desc="learned partitioner vs plain GPU radix, 10^9 keys: lognormal mixture (sigma 0.5/1/2) with "
+                     "5 % Zipf(1.1) hot values, as u64 and as f64; untrained (uniform-knot) model as the ablation",
+                ok=all(v["ok"] for v in res.values()), **res)
+
+
 def pmc_traffic(workload: str, log2n: int):
     """HBM bytes per k_sweep launch from the committed rocprofv3 counter runs.
 
@@ -347,6 +457,8 @@ def main():
             r = run_segmented(max(3, a.steps // 2), ctx)
         elif name == "by_key":
             r = run_by_key(max(3, a.steps // 3), ctx)
+        elif name == "learned":
+            r = run_learned(max(2, a.steps // 5), ctx)
         else:
             r = run_workload(name, a.log2n, max(3, a.steps // 3), 1, ctx)
             r["roofline_frac"] = r["total_gbs"] / HBM_PEAK_GBS

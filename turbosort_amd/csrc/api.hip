@@ -574,7 +574,7 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   size_t b = 0;
   auto take = [&](int i, size_t bytes) { off[i] = b; b += up(bytes); };
   take(0, sizeof(ts::SortMeta));
-  take(1, ts::status_bytes(n, kb));
+  take(1, ts::status_bytes(n, kb, true));
   take(2, n * size_t(kb));
   take(3, S * 8);
   take(4, S * 8);

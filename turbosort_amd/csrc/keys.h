@@ -159,25 +159,31 @@ struct SortMeta {
 };
 
 // ---------------------------------------------------------------------------
-// Learned partition model: a monotone piecewise-linear CDF over a
-// logarithmic key coordinate, evaluated in integer arithmetic so that the
-// bucket of a key never decreases as the key grows.
+// Learned partition model: a monotone piecewise-linear CDF over a key
+// coordinate, evaluated in integer arithmetic so that the bucket of a key
+// never decreases as the key grows.
 //
-// log2_fixed(k) = e * 2^23 + (the 23 bits below k's leading one), e its
-// position: non-decreasing in k.  The sample's range [xlo, xhi] of that
-// coordinate is stretched onto u in [0, kKnots * 2^19); knot i covers
-// u >> 19 == i, and F(u) = knots[i] + (knots[i+1] - knots[i]) * (u mod 2^19) / 2^19
-// (rounded down) never exceeds knots[i+1], so F is non-decreasing across
-// knots too.  A key's bucket is F >> bucket_shift.
+// The coordinate of a radix key k is d = max(k - kmin, 0) (kmin: the sample's
+// smallest key), then either log2_fixed(d) -- for integer keys spread over
+// many orders of magnitude -- or d >> shift (linear; float keys, whose bit
+// patterns already grow with the logarithm of the value).  Both are
+// non-decreasing in k.  log2_fixed(d) = e * 2^23 + (the 23 bits below d's
+// leading one), e its position.  The sample's coordinate range [0, xhi] is
+// stretched onto u in [0, kKnots * 2^19); knot i covers u >> 19 == i, and
+// F(u) = knots[i] + (knots[i+1] - knots[i]) * (u mod 2^19) / 2^19 (rounded
+// down) never exceeds knots[i+1], so F is non-decreasing across knots too.
+// A key's bucket is F >> bucket_shift.
 constexpr int kKnots = 1024;
 constexpr int kKnotFracBits = 19;  // u bits below the knot index
 
 struct LearnedModel {
   uint32_t knots[kKnots + 1];  // F at each knot, 32-bit fixed point, non-decreasing
   uint32_t bucket_shift;       // 32 - log2(number of buckets)
-  uint32_t pad;
-  unsigned long long xlo, xhi;  // coordinate range of the sample
-  unsigned long long mul;       // floor(2^61 / (xhi - xlo + 1))
+  uint32_t linear;             // 1: coordinate d >> shift; 0: log2_fixed(d)
+  uint32_t shift;
+  unsigned long long kmin;  // smallest sample key
+  unsigned long long xhi;   // coordinate of the largest sample key
+  unsigned long long mul;   // floor(2^61 / (xhi + 1))
 };
 
 __host__ __device__ __forceinline__ unsigned long long log2_fixed(unsigned long long k) {
@@ -191,11 +197,16 @@ __host__ __device__ __forceinline__ unsigned long long log2_fixed(unsigned long 
   return (unsigned long long)e << 23 | (frac & 0x7FFFFFull);
 }
 
+__host__ __device__ __forceinline__ unsigned long long learned_x(unsigned long long key, const LearnedModel& m) {
+  const unsigned long long d = key > m.kmin ? key - m.kmin : 0ull;
+  return m.linear ? d >> m.shift : log2_fixed(d);
+}
+
 // The model coordinate of a radix key: u in [0, kKnots << kKnotFracBits).
 __host__ __device__ __forceinline__ uint32_t learned_coord(unsigned long long key, const LearnedModel& m) {
-  unsigned long long x = log2_fixed(key);
-  x = x < m.xlo ? m.xlo : (x > m.xhi ? m.xhi : x);
-  return uint32_t(((x - m.xlo) * m.mul) >> 32);
+  unsigned long long x = learned_x(key, m);
+  x = x > m.xhi ? m.xhi : x;
+  return uint32_t((x * m.mul) >> 32);
 }
 
 __host__ __device__ __forceinline__ uint32_t learned_cdf(uint32_t u, const uint32_t* knots) {

@@ -47,21 +47,25 @@ __global__ void k_learn_sample(const U* __restrict__ data, size_t n, int mode, u
 // T evenly spaced sample keys, their coordinate u (in knot units) and their
 // CDF target (rank + 1/2) / S.
 __global__ void k_learn_points(const unsigned long long* __restrict__ sorted, size_t S, LearnedModel* model,
-                               float* __restrict__ u, float* __restrict__ t, uint32_t T, uint32_t bucket_shift) {
+                               float* __restrict__ u, float* __restrict__ t, uint32_t T, uint32_t bucket_shift,
+                               int linear) {
   __shared__ LearnedModel m;
   if (threadIdx.x == 0) {
-    const unsigned long long xlo = log2_fixed(sorted[0]), xhi = log2_fixed(sorted[S - 1]);
-    m.xlo = xlo;
-    m.xhi = xhi;
-    m.mul = (1ull << 61) / (xhi - xlo + 1);
+    m.kmin = sorted[0];
+    m.linear = uint32_t(linear);
+    const unsigned long long span = sorted[S - 1] - sorted[0];
+    const int bits = span ? 64 - __clzll((long long)span) : 0;
+    m.shift = linear && bits > 32 ? uint32_t(bits - 32) : 0u;
+    m.xhi = learned_x(sorted[S - 1], m);
+    m.mul = (1ull << 61) / (m.xhi + 1);
     m.bucket_shift = bucket_shift;
-    m.pad = 0;
     if (blockIdx.x == 0) {
-      model->xlo = m.xlo;
+      model->kmin = m.kmin;
+      model->linear = m.linear;
+      model->shift = m.shift;
       model->xhi = m.xhi;
       model->mul = m.mul;
       model->bucket_shift = bucket_shift;
-      model->pad = 0;
     }
   }
   __syncthreads();
@@ -371,7 +375,10 @@ __global__ void k_pwl_uniform(LearnedModel* model) {
 }
 
 // Bucket bounds of keys grouped by bucket: offsets[b] = first position whose
-// bucket is >= b, offsets[B] = n.
+// bucket is >= b, offsets[B] = n.  Each lane takes kBoundRun consecutive keys
+// and evaluates the model once per key (plus once for the key before its run).
+constexpr int kBoundRun = 8;
+
 template <class U>
 __global__ void k_learn_bounds(const U* __restrict__ data, size_t n, int mode, const LearnedModel* __restrict__ model,
                                unsigned long long* __restrict__ offsets, uint32_t B) {
@@ -379,14 +386,26 @@ __global__ void k_learn_bounds(const U* __restrict__ data, size_t n, int mode, c
   for (unsigned i = threadIdx.x; i <= unsigned(kKnots); i += blockDim.x) knots[i] = model->knots[i];
   __syncthreads();
   const LearnedModel& m = *model;
-  auto bucket = [&](size_t i) {
-    return learned_cdf(learned_coord((unsigned long long)to_key<U>(data[i], mode), m), knots) >> m.bucket_shift;
+  auto bucket = [&](U x) {
+    return learned_cdf(learned_coord((unsigned long long)to_key<U>(x, mode), m), knots) >> m.bucket_shift;
   };
+  const size_t runs = (n + kBoundRun - 1) / kBoundRun;
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
-  for (size_t i = g; i <= n; i += gs) {
-    const uint32_t prev = i == 0 ? 0u : bucket(i - 1) + 1;  // first bucket starting at i
-    const uint32_t here = i == n ? B : bucket(i);
-    for (uint32_t b = prev; b <= here; ++b) offsets[b] = i;
+  for (size_t r = g; r < runs; r += gs) {
+    const size_t i0 = r * kBoundRun;
+    U x[kBoundRun];
+#pragma unroll
+    for (int k = 0; k < kBoundRun; ++k) x[k] = data[min(i0 + k, n - 1)];
+    uint32_t prev = i0 == 0 ? 0u : bucket(data[i0 - 1]) + 1;  // first bucket that may start at i0
+#pragma unroll
+    for (int k = 0; k < kBoundRun; ++k) {
+      if (i0 + k >= n) break;
+      const uint32_t here = bucket(x[k]);
+      for (uint32_t b = prev; b <= here; ++b) offsets[b] = i0 + k;
+      prev = here + 1;
+    }
+    if (i0 + kBoundRun >= n)  // this run holds the last key
+      for (uint32_t b = prev; b <= B; ++b) offsets[b] = n;
   }
 }
 
@@ -431,8 +450,9 @@ template <class U> hipError_t run_learned(const LearnedJob& j, hipStream_t s, Ke
                                       : small_sort(8, kIdentity, j.sample_a, j.sample, nullptr, 0, 0, nullptr, s, clk);
   if (e != hipSuccess) return e;
   const unsigned pg = unsigned(std::min<size_t>((j.points + 255) / 256, 1024));
+  // Float keys: their bit patterns already follow the logarithm of the value.
   hipLaunchKernelGGL(k_learn_points, dim3(pg), dim3(256), 0, s, j.sample_a, j.sample, j.model, j.u, j.t, j.points,
-                     j.bucket_shift);
+                     j.bucket_shift, int(j.mode == kFloat));
   if (j.train) {
     hipLaunchKernelGGL(k_knot_first, dim3((kKnots + 256) / 256), dim3(256), 0, s, j.sample_a, j.sample, j.model,
                        j.knot_first);
@@ -461,7 +481,7 @@ template <class U> hipError_t run_learned(const LearnedJob& j, hipStream_t s, Ke
   p.model = j.model;
   e = radix_sort(p, s, clk);
   if (e != hipSuccess) return e;
-  const unsigned bg = unsigned(std::min<size_t>((j.n + 256) / 256, size_t(j.num_cus) * 16));
+  const unsigned bg = unsigned(std::min<size_t>((j.n / kBoundRun + 256) / 256, size_t(j.num_cus) * 16));
   if (clk) clk->begin(s, "k_learn_bounds", double(j.n) * sizeof(U));
   hipLaunchKernelGGL((k_learn_bounds<U>), dim3(bg), dim3(256), 0, s, static_cast<const U*>(j.data), j.n, j.mode,
                      j.model, j.offsets, j.buckets);

@@ -11,7 +11,7 @@ namespace ts {
 // Records the duration of each kernel of a sort with HIP events on the sort's
 // stream, when enabled on the context.
 struct KernelClock {
-  static constexpr int kMax = 64;
+  static constexpr int kMax = 256;
   bool on = false;
   int count = 0;
   const char* name[kMax];
@@ -50,23 +50,29 @@ struct KernelClock {
 #ifndef TS_LOOKBACK_WINDOW
 #define TS_LOOKBACK_WINDOW 4
 #endif
-template <int KB> struct SweepShape {
+// The learned partition's 8-byte tiles are smaller: the staged digits and the
+// model's knots also live in LDS, and two workgroups must still fit a CU.
+#ifndef TS_SWEEP_IPT_WIDE_LEARNED
+#define TS_SWEEP_IPT_WIDE_LEARNED 14
+#endif
+template <int KB, bool LEARNED = false> struct SweepShape {
   static constexpr int threads = TS_SWEEP_THREADS;
-  static constexpr int ipt = KB >= 8 ? TS_SWEEP_IPT_WIDE : TS_SWEEP_IPT_NARROW;
+  static constexpr int ipt = KB >= 8 ? (LEARNED ? TS_SWEEP_IPT_WIDE_LEARNED : TS_SWEEP_IPT_WIDE) : TS_SWEEP_IPT_NARROW;
   static constexpr int tile = threads * ipt;
 };
 
-inline size_t sweep_tile(int key_bytes) {
-  return key_bytes >= 8 ? size_t(SweepShape<8>::tile) : size_t(SweepShape<4>::tile);
+inline size_t sweep_tile(int key_bytes, bool learned = false) {
+  if (key_bytes < 8) return size_t(SweepShape<4>::tile);
+  return learned ? size_t(SweepShape<8, true>::tile) : size_t(SweepShape<8>::tile);
 }
 // Upper bound on the tiles of one pass: every segment may end in a partial tile.
-inline uint32_t sweep_tiles(size_t n, int key_bytes) {
-  const size_t tile = sweep_tile(key_bytes);
+inline uint32_t sweep_tiles(size_t n, int key_bytes, bool learned = false) {
+  const size_t tile = sweep_tile(key_bytes, learned);
   return uint32_t((n + tile - 1) / tile + kSegments);
 }
 // Length of the first pass's position segments (a whole number of tiles).
-inline size_t first_segment(size_t n, int key_bytes) {
-  const size_t tile = sweep_tile(key_bytes);
+inline size_t first_segment(size_t n, int key_bytes, bool learned = false) {
+  const size_t tile = sweep_tile(key_bytes, learned);
   const size_t per = (n + kSegments - 1) / kSegments;
   return ((per + tile - 1) / tile) * tile;
 }
@@ -74,8 +80,8 @@ inline size_t first_segment(size_t n, int key_bytes) {
 // Look-back words are 32-bit (2 flag bits, 30 value bits) while every count
 // fits in 30 bits, else 64-bit.
 inline bool wide_status(size_t n) { return n >= (size_t(1) << 30); }
-inline size_t status_bytes(size_t n, int key_bytes) {
-  return size_t(2) * sweep_tiles(n, key_bytes) * 256 * (wide_status(n) ? 8 : 4);
+inline size_t status_bytes(size_t n, int key_bytes, bool learned = false) {
+  return size_t(2) * sweep_tiles(n, key_bytes, learned) * 256 * (wide_status(n) ? 8 : 4);
 }
 
 // One LSD sort on the device.  Keys start in keys_a (plain sort) or are read

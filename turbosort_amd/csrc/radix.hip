@@ -380,8 +380,8 @@ __device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[(
 // order to final_vals.
 // LDS of one scatter tile, declared once in the kernel.
 template <class U, bool VALS, int IPT, bool LEARNED = false> struct SweepLds {
-  static constexpr int NW = SweepShape<sizeof(U)>::threads / 64;
-  static constexpr int TILE = SweepShape<sizeof(U)>::threads * IPT;
+  static constexpr int NW = SweepShape<sizeof(U), LEARNED>::threads / 64;
+  static constexpr int TILE = SweepShape<sizeof(U), LEARNED>::threads * IPT;
   uint32_t whist[NW][256];  // per-wave digit counts, then staging bases
   U keys[TILE];
   uint32_t vals[VALS ? TILE : 1];
@@ -397,12 +397,13 @@ template <class U, bool VALS, int IPT, bool LEARNED = false> struct SweepLds {
 // counts, rank, look back, stage, store.  FULL tiles (all but each segment's
 // last) run without any per-key bounds test.
 template <class U, bool VALS, class S, bool FULL, bool LEARNED>
-__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>::ipt, LEARNED>& L, const SweepArgs<U>& a,
+__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED>::ipt, LEARNED>& L,
+                                           const SweepArgs<U>& a,
                                            S* __restrict__ status, int pass, unsigned shift, bool first, bool last,
                                            size_t tile_base, uint32_t valid, size_t first_slot, size_t slot,
                                            bool first_round, unsigned long long digit_base) {
-  constexpr int IPT = SweepShape<sizeof(U)>::ipt;
-  constexpr int THREADS = SweepShape<sizeof(U)>::threads;
+  constexpr int IPT = SweepShape<sizeof(U), LEARNED>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U), LEARNED>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
   constexpr int VB = Flags<S>::kBits;
@@ -599,9 +600,10 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U)>
 // pass applies the inverse key transform, and in a by-key sort writes only the
 // order to final_vals.
 template <class U, bool VALS, class S, bool LEARNED = false>
-__global__ __launch_bounds__(SweepShape<sizeof(U)>::threads) void k_sweep(SweepArgs<U> a, S* __restrict__ status, int pass) {
-  constexpr int IPT = SweepShape<sizeof(U)>::ipt;
-  constexpr int THREADS = SweepShape<sizeof(U)>::threads;
+__global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED>::threads)) void k_sweep(SweepArgs<U> a,
+                                                                                  S* __restrict__ status, int pass) {
+  constexpr int IPT = SweepShape<sizeof(U), LEARNED>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U), LEARNED>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
   __shared__ SweepLds<U, VALS, IPT, LEARNED> L;
@@ -669,7 +671,7 @@ __global__ void k_iota_if_dead(const SortMeta* m, uint32_t* out, size_t n) {
 }
 
 template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk) {
-  const uint32_t tiles = sweep_tiles(j.n, sizeof(U));
+  const uint32_t tiles = sweep_tiles(j.n, sizeof(U), LEARNED);
   SweepArgs<U> a;
   a.keys_a = static_cast<U*>(j.keys_a);
   a.keys_b = static_cast<U*>(j.keys_b);
@@ -690,7 +692,7 @@ template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& 
   for (int p = 0; p < DigitSource<U, LEARNED>::P; ++p) {
     if (clk) clk->begin(s, "k_sweep", bytes);
     if constexpr (LEARNED)
-      hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a,
+      hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U), true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
     else if (vals)
       hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
@@ -707,12 +709,12 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, s, j.meta, P);
   // Histogram: blocks_per_seg workgroups per position segment, each over a
   // contiguous chunk of it.
-  const size_t seg0 = first_segment(j.n, KB);
+  const size_t seg0 = first_segment(j.n, KB, LEARNED);
   const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * 4 / kSegments));
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
   size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
   per_block = std::max(step, ((per_block + step - 1) / step) * step);
-  const uint32_t tiles = sweep_tiles(j.n, KB);
+  const uint32_t tiles = sweep_tiles(j.n, KB, LEARNED);
   const size_t zero_words = size_t(tiles) * 256 * (wide_status(j.n) ? 2 : 1);
   const dim3 hgrid(blocks_per_seg * kSegments);
   if (clk) clk->begin(s, "k_hist", double(j.n) * (j.records ? double(j.stride) + KB : KB));
@@ -729,7 +731,7 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
                        size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
                        static_cast<uint32_t*>(j.status), zero_words, nullptr);
   if (clk) clk->end(s);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(KB)));
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(KB, LEARNED)));
   hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk)
                                   : run_sweeps<U, uint32_t, LEARNED>(j, s, clk);
   if (e != hipSuccess) return e;
