@@ -21,20 +21,6 @@
 namespace ts {
 namespace {
 
-// Exclusive scan of one value per lane over the first 256 lanes of the
-// workgroup (every lane of the workgroup must call it; s_wave holds one slot
-// per wave).
-template <class T> __device__ __forceinline__ T block256_exclusive_add(T v, T* s_wave) {
-  const unsigned tid = threadIdx.x, w = tid >> 6;
-  T inc = wave_inclusive_add(v);
-  if ((tid & 63) == 63) s_wave[w] = inc;
-  __syncthreads();
-  T off = 0;
-  for (unsigned i = 0; i < w && i < 4; ++i) off += s_wave[i];
-  __syncthreads();
-  return off + inc - v;
-}
-
 template <class U> __device__ __forceinline__ U first_lane(U x) {
   if constexpr (sizeof(U) == 8) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
@@ -292,75 +278,6 @@ template <class U> struct SweepArgs {
 
 // About two seconds of s_sleep(1) polling.
 constexpr uint32_t kSpinLimit = 1u << 26;
-
-// The byte digit of a key at bit `shift` (a multiple of 8, uniform).
-template <class U> __device__ __forceinline__ uint32_t digit_of(U k, unsigned shift) {
-  if constexpr (sizeof(U) == 8) {
-    const uint32_t part = (shift & 32u) ? uint32_t(uint64_t(k) >> 32) : uint32_t(k);
-    return __builtin_amdgcn_ubfe(part, shift & 31u, 8u);
-  } else {
-    return __builtin_amdgcn_ubfe(uint32_t(k), shift, 8u);
-  }
-}
-
-// x & ~(vote ^ m): keep the lanes whose vote equals this lane's bit, as one
-// v_bitop3_b32 (truth table 0x90 over x, vote, m).  A builtin, not inline
-// asm, so the compiler schedules the wait states a VALU read of the freshly
-// written vote (an SGPR) needs.
-__device__ __forceinline__ uint32_t keep_equal(uint32_t x, uint32_t vote, uint32_t m) {
-  return __builtin_amdgcn_bitop3_b32(x, vote, m, 0x90);
-}
-
-// Stable rank of each of a lane's IPT keys among the keys of its wave that
-// carry the same digit, counted into the wave's LDS histogram.  Lanes with
-// equal digits are found with eight ballots: lane l keeps the lanes whose
-// vote on every digit bit equals its own, i.e. ~(vote ^ mybit) per bit.
-template <class U, int IPT, bool FULL>
-__device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[(IPT + 1) / 2], unsigned shift,
-                                             uint32_t* whist, uint32_t lb, uint32_t valid) {
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint32_t d = digit_of(k[j], shift);
-    uint32_t lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
-    bool ok = true;
-    if (!FULL) {
-      ok = lb + uint32_t(j) * 64 < valid;
-      const uint64_t okm = __ballot(ok);
-      lo = uint32_t(okm);
-      hi = uint32_t(okm >> 32);
-    }
-    // All eight votes first, then the combines: a vote (an SGPR) read by the
-    // VALU right after the compare that wrote it costs wait states.
-    uint32_t m[8];
-    uint64_t vote[8];
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      m[b] = uint32_t(__builtin_amdgcn_sbfe(int32_t(d), uint32_t(b), 1u));  // 0 or ~0
-      // Emits nothing; keeps the compiler from re-deriving the vote from d
-      // with a second shift instead of comparing m.
-      asm("" : "+v"(m[b]));
-      vote[b] = __ballot(m[b] != 0);
-    }
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      lo = keep_equal(lo, uint32_t(vote[b]), m[b]);
-      hi = keep_equal(hi, uint32_t(vote[b] >> 32), m[b]);
-    }
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-    const uint32_t before = whist[d];
-    // Every lane of a digit group writes the same new count, so no branch is
-    // needed.  A masked-off lane of the last tile holds a copy of a real key;
-    // its group holds only the valid lanes sharing that digit, so it writes
-    // their count (or the unchanged count when there are none).
-    whist[d] = before + uint32_t(__popc(lo) + __popc(hi));
-    // Ranks are below 64 * IPT, so two share a register.
-    if (j & 1)
-      rk[j >> 1] |= (before + below) << 16;
-    else
-      rk[j >> 1] = before + below;
-    (void)ok;
-  }
-}
 
 // One stable scatter pass by the byte digit meta->live[pass] (scatter.rs:26-40).
 //

@@ -402,6 +402,146 @@ template <class U, int T, int IPT, bool VALS> __global__ __launch_bounds__(T) vo
 }
 
 // ---------------------------------------------------------------------------
+// LDS radix sort with byte digits for the longer classes: each wave ranks its
+// keys stably by digit with ballots (rank_in_wave), a scan over (digit, wave)
+// counts places them, and the pass scatters inside LDS.  Wave w holds the
+// positions w*64*IPT .. (w+1)*64*IPT-1, lane-minor, so the order (wave, step,
+// lane) is the position order and the pass is stable.  Half the passes of the
+// nibble kernel, for about 40 VALU operations per key and pass.
+
+template <class U, int T, int IPT, bool VALS> struct SegByteLds {
+  static constexpr int CAP = T * IPT;
+  alignas(16) U keys[CAP];
+  alignas(16) uint32_t vals[VALS ? CAP : 1];
+  uint32_t whist[T / 64][256];  // per-wave digit counts, then bases
+  uint32_t wsum[4];
+  uint32_t any[2];
+};
+
+template <class U, int T, int IPT, bool VALS>
+__global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
+  static_assert(T >= 256 && T % 64 == 0, "the first 256 lanes carry one digit each");
+  static_assert(64 * IPT <= 65536, "wave ranks are 16-bit");
+  constexpr int NW = T / 64;
+  constexpr int P = int(sizeof(U));
+  __shared__ SegByteLds<U, T, IPT, VALS> L;
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t wb = w * 64 * IPT + lane;  // this lane's slot for step j: wb + 64 j
+  const uint32_t count = a.counts[cls];
+  const uint32_t* __restrict__ list = a.lists + size_t(cls) * a.n_segments;
+
+  for (uint32_t job = blockIdx.x; job < count; job += gridDim.x) {
+    const uint32_t seg = list[job];
+    const size_t lo = a.offsets[seg];
+    const uint32_t len = uint32_t(seg_end(a, seg) - lo);
+    U* __restrict__ keys = static_cast<U*>(a.keys) + lo;
+    uint32_t* __restrict__ vals = VALS ? a.vals + lo : nullptr;
+
+    // Striped, coalesced load (indices clamped so all loads issue at once);
+    // padding keys are the largest key and, after every real key, stay last.
+    {
+      U kin[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) kin[j] = keys[min(uint32_t(j) * T + tid, len - 1)];
+      if (VALS) {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) L.vals[uint32_t(j) * T + tid] = vals[min(uint32_t(j) * T + tid, len - 1)];
+      }
+      to_keys(kin, a.mode);
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t i = uint32_t(j) * T + tid;
+        L.keys[i] = i < len ? kin[j] : U(~U(0));
+      }
+    }
+    if (tid == 0) {
+      L.any[0] = 0;
+      L.any[1] = 0;
+    }
+    __syncthreads();
+
+    U k[IPT];
+    uint32_t v[VALS ? IPT : 1];
+    const U k0 = L.keys[0];
+    U vary = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t p = wb + 64u * j;
+      k[j] = L.keys[p];
+      if (VALS) v[j] = L.vals[p];
+      if (p < len) vary |= U(k[j] ^ k0);
+    }
+    {
+      const uint64_t x = wave_or<uint64_t>(uint64_t(vary));
+      if (lane == 0 && x) {
+        atomicOr(&L.any[0], uint32_t(x));
+        if (sizeof(U) == 8) atomicOr(&L.any[1], uint32_t(x >> 32));
+      }
+    }
+    __syncthreads();
+    const uint64_t varying = uint64_t(L.any[0]) | (uint64_t(L.any[1]) << 32);
+    // With every key equal the stable order is the input order: nothing moves.
+    if (varying == 0) {
+      __syncthreads();  // every lane has read L.any before the next segment clears it
+      continue;
+    }
+
+#pragma unroll 1
+    for (int p = 0; p < P; ++p) {
+      const unsigned shift = 8u * unsigned(p);
+      if (((varying >> shift) & 0xFFu) == 0) continue;  // every key agrees on this digit
+      for (unsigned x = lane; x < 256; x += 64) L.whist[w][x] = 0;  // this wave's row only
+      uint32_t rk[(IPT + 1) / 2];
+      rank_in_wave<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) opaque(k[j]);  // recompute digits below rather than keep them
+      __syncthreads();
+      // Digit bases: digits in order, then waves in order.
+      uint32_t cnt = 0;
+      if (tid < 256) {
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) cnt += L.whist[ww][tid];
+      }
+      uint32_t run = block256_exclusive_add(cnt, L.wsum);
+      if (tid < 256) {
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) {
+          const uint32_t c = L.whist[ww][tid];
+          L.whist[ww][tid] = run;
+          run += c;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t d = digit_of(k[j], shift);
+        const uint32_t dst = L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
+        L.keys[dst] = k[j];
+        if (VALS) L.vals[dst] = v[j];
+        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        k[j] = L.keys[wb + 64u * j];
+        if (VALS) v[j] = L.vals[wb + 64u * j];
+      }
+    }
+
+    // L.keys / L.vals hold the sorted segment; write it out striped.
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = uint32_t(j) * T + tid;
+      if (i < len) {
+        keys[i] = from_key<U>(L.keys[i], a.mode);
+        if (VALS) vals[i] = L.vals[i];
+      }
+    }
+    __syncthreads();  // the next segment reuses the LDS
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Three-way split of long keys-only segments around a pivot key (equal keys
 // are equal bits, so no order among them is observable): keys below the
 // pivot go to the front of the segment, keys above to the back, and the
@@ -558,22 +698,23 @@ template <class U> hipError_t run_split(const SplitJob& j, hipStream_t s) {
 struct ClassShape {
   int bitonic_w;  // > 0: wave network of this width
   int threads, ipt;
+  bool bytes;  // byte digits (k_seg_radix8) instead of nibbles
 };
 
 template <class U, bool VALS> constexpr ClassShape class_shape(int c) {
   // 8-byte keys with values: 16384 keys would need 192 KB of LDS.
   constexpr bool wide = sizeof(U) == 8;
   switch (c) {
-    case 0: return {16, 0, 0};
-    case 1: return {32, 0, 0};
-    case 2: return {64, 0, 0};
-    case 3: return {0, 64, 4};
-    case 4: return {0, 64, 16};
-    case 5: return {0, 128, 16};
-    case 6: return {0, 256, 16};
-    case 7: return {0, 512, 16};
-    case 8: return wide && VALS ? ClassShape{0, 0, 0} : ClassShape{0, 512, 32};
-    default: return {0, 0, 0};
+    case 0: return {16, 0, 0, false};
+    case 1: return {32, 0, 0, false};
+    case 2: return {64, 0, 0, false};
+    case 3: return {0, 64, 4, false};
+    case 4: return {0, 64, 16, false};
+    case 5: return {0, 128, 16, false};
+    case 6: return {0, 256, 16, true};
+    case 7: return {0, 512, 16, true};
+    case 8: return wide && VALS ? ClassShape{0, 0, 0, false} : ClassShape{0, 1024, 16, true};
+    default: return {0, 0, 0, false};
   }
 }
 
@@ -588,6 +729,15 @@ void launch_class(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk
   if constexpr (sh.bitonic_w > 0) {
     if (clk) clk->begin(s, "k_seg_bitonic", 0);
     hipLaunchKernelGGL((k_seg_bitonic<U, sh.bitonic_w, VALS>), dim3(num_cus * 8), dim3(256), 0, s, a, C);
+    if (clk) clk->end(s);
+  } else if constexpr (sh.threads > 0 && sh.bytes) {
+    constexpr size_t lds = sizeof(SegByteLds<U, sh.threads, sh.ipt, VALS>);
+    constexpr int by_lds = int((160 * 1024) / lds);
+    constexpr int by_threads = 2048 / sh.threads;
+    constexpr int per_cu = std::max(1, std::min(by_lds, by_threads));
+    if (clk) clk->begin(s, "k_seg_radix8", 0);
+    hipLaunchKernelGGL((k_seg_radix8<U, sh.threads, sh.ipt, VALS>), dim3(num_cus * per_cu), dim3(sh.threads), 0, s,
+                       a, C);
     if (clk) clk->end(s);
   } else if constexpr (sh.threads > 0) {
     constexpr size_t lds = sizeof(SegRadixLds<U, sh.threads, sh.ipt, VALS>);
