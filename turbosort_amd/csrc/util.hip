@@ -80,7 +80,22 @@ __global__ void k_check(const U* __restrict__ x, size_t n, int mode, SortMeta* m
     xr ^= __shfl_xor(xr, d, 64);
     bad += __shfl_xor(bad, d, 64);
   }
+  // One set of atomics per workgroup: the grid is a few workgroups per CU,
+  // so the three global words see little contention.
+  __shared__ unsigned long long part[3][kThreads / 64];
+  const unsigned w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    part[0][w] = sum;
+    part[1][w] = xr;
+    part[2][w] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (unsigned i = 1; i < kThreads / 64; ++i) {
+      sum += part[0][i];
+      xr ^= part[1][i];
+      bad += part[2][i];
+    }
     atomicAdd(&m->check[0], sum);
     atomicXor(&m->check[1], xr);
     atomicAdd(&m->check[2], bad);
@@ -129,7 +144,7 @@ hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStre
 hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, SortMeta* meta, hipStream_t s) {
   hipLaunchKernelGGL(k_check_zero, dim3(1), dim3(64), 0, s, meta);
   if (n == 0) return hipGetLastError();
-  const unsigned grid = grid_for(n, 8);
+  const unsigned grid = std::min(grid_for(n, 8), 2048u);
   switch (key_bytes) {
     case 1:
       hipLaunchKernelGGL(k_check<uint8_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint8_t*>(data), n,
