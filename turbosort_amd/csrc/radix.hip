@@ -58,8 +58,23 @@ __global__ void k_init(SortMeta* m, int P) {
 // instead of 64 colliding atomics.  The grid also clears the first look-back
 // row.
 constexpr int kHistThreads = 256;
+#ifndef TS_HIST_UNROLL_NARROW
+#define TS_HIST_UNROLL_NARROW 16
+#endif
+#ifndef TS_HIST_UNROLL_WIDE
+#define TS_HIST_UNROLL_WIDE 16
+#endif
+#ifndef TS_HIST_BLOCKS_PER_CU
+#define TS_HIST_BLOCKS_PER_CU 4
+#endif
+#ifndef TS_HIST_BLOCKS_PER_CU_WIDE
+#define TS_HIST_BLOCKS_PER_CU_WIDE 2
+#endif
+// Measured on MI355X at 2^28 keys: 16 keys per lane per step; 8-byte keys
+// (eight digit tables per workgroup) prefer fewer, longer workgroups.
 template <class U> struct HistUnroll {
-  static constexpr int value = sizeof(U) >= 8 ? 8 : 16;
+  static constexpr int value = sizeof(U) >= 8 ? TS_HIST_UNROLL_WIDE : TS_HIST_UNROLL_NARROW;
+  static constexpr int blocks_per_cu = sizeof(U) >= 8 ? TS_HIST_BLOCKS_PER_CU_WIDE : TS_HIST_BLOCKS_PER_CU;
 };
 
 // The digits of a pass come from the radix key itself, or (LEARNED) from the
@@ -627,7 +642,7 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   // Histogram: blocks_per_seg workgroups per position segment, each over a
   // contiguous chunk of it.
   const size_t seg0 = first_segment(j.n, KB, LEARNED);
-  const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * 4 / kSegments));
+  const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * HistUnroll<U>::blocks_per_cu / kSegments));
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
   size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
   per_block = std::max(step, ((per_block + step - 1) / step) * step);
