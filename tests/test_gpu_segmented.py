@@ -160,3 +160,36 @@ def test_long_keys_only_segments_split(gpu, dtype):
     keys = np.concatenate([a, b, c, random_of(dtype, 300, seed=59)])
     off = offsets_of([n_hot, 150_000, 90_000, 300])
     check(keys, off, None, gpu)
+
+
+def test_graph_capture_replays(gpu):
+    """With a bound that fits one workgroup the sort makes no host call, so
+    it can be captured in a graph; every replay sorts fresh input exactly
+    (the input is restored eagerly between replays)."""
+    off, keys, vals = cf.segmented_workload(7, 2000)
+    tk = to_dev(keys, gpu)
+    tv = to_dev(vals, gpu)
+    to = to_dev(off, gpu)
+    src_k, src_v = tk.clone(), tv.clone()
+    want_k, want_v = cf.segmented_sort(keys, off, vals)
+    ctx = ts.Context(gpu)
+    ts.segmented_sort(tk, to, tv, max_segment_len=16384, ctx=ctx)  # grows the workspace before capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        tk.copy_(src_k)
+        tv.copy_(src_v)
+        with torch.cuda.graph(g, stream=s):
+            ts.segmented_sort(tk, to, tv, max_segment_len=16384, ctx=ctx)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        tk.copy_(src_k)
+        tv.copy_(src_v)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(to_host(tk, np.uint32), want_k)
+        assert np.array_equal(to_host(tv, np.uint32), want_v)

@@ -295,3 +295,19 @@ def test_lognormal_mixture_generator():
     assert counts.max() > 0.005 * x.size
     f = cf.lognormal_mixture_keys(8, 1000, as_f64=True)
     assert f.dtype == np.float64 and np.all(f > 0)
+
+
+def test_bench_generators_match_the_oracle_generators():
+    """bench.py builds its workloads without the oracle; they must be the
+    same inputs the parity tests use."""
+    import bench
+
+    off, _, _ = cf.segmented_workload(7, 5000)
+    assert np.array_equal(bench.segment_offsets(7, 5000), off.astype(np.int64))
+    assert np.array_equal(bench.np_splitmix64(9, 1000), cf.splitmix64(9, 1000))
+    rec = bench.score_records(20_000, 5)
+    assert np.array_equal(rec["id"], np.arange(20_000, dtype=np.uint32))
+    scores = rec["score"].view(np.float32)
+    nan = np.isnan(scores)
+    assert 0.001 < nan.mean() < 0.01  # two of the five specials, 1 % of the records
+    assert np.isinf(scores).any() and (rec["score"] == 0x80000000).any()
