@@ -202,8 +202,11 @@ def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7):
     import torch
 
     import turbosort_amd as ts
+    from turbosort_amd import autotune
 
     dev = torch.device("cuda", 0)
+    # Kernel shape per size class, picked on this GPU before the timed runs.
+    tuning = autotune.tune_segmented(torch.uint32, True, ctx=ctx)
     off = torch.from_numpy(segment_offsets(seed, n_segments)).to(dev)
     n = int(off[-1])
     src = torch.empty(n, dtype=torch.uint32, device=dev)
@@ -253,7 +256,8 @@ def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7):
                      "state 7), u32 keys uniform, u32 values = global index", n=n, segments=n_segments,
                 ms_per_step=dt * 1e3, pairs_per_s=n / dt, algo_bytes_per_pair=16,
                 algorithmic_GBps=16 * n / dt / 1e9, roofline_frac=16 * n / dt / 1e9 / HBM_PEAK_GBS,
-                graph_replay_ms_median=times[len(times) // 2], ok=ok)
+                graph_replay_ms_median=times[len(times) // 2],
+                autotuned={str(c): tuning["names"][c][v] for c, v in tuning["choice"].items()}, ok=ok)
 
 
 def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5):

@@ -105,6 +105,16 @@ ts_status ts_segmented_sort_pairs(ts_ctx* ctx, ts_dtype dtype, void* d_keys, uin
  * without values); longer segments take the device radix sort. */
 size_t ts_segmented_lds_max_len(ts_dtype dtype, int with_values);
 
+/* Autotuning hooks.  Size class c (0 .. 8: segments of up to 16, 32, 64, 256,
+ * 1024, 2048, 4096, 8192, 16384 keys) can run one of
+ * ts_segmented_variants(...) kernel shapes, named by
+ * ts_segmented_variant_name(...); variant 0 is the default.  The choice is
+ * kept per context and per key width (with / without values); the learned
+ * sort's bucket sorts use the keys-only choice. */
+int ts_segmented_variants(ts_dtype dtype, int with_values, int size_class);
+const char* ts_segmented_variant_name(ts_dtype dtype, int with_values, int size_class, int variant);
+ts_status ts_segmented_set_variant(ts_ctx* ctx, ts_dtype dtype, int with_values, int size_class, int variant);
+
 /* Learned partitioner.  No reference counterpart: the output equals
  * ts_sort's (turbosort::sort, lib.rs:99-101) for 4- and 8-byte key types,
  * reached by bucketing the keys with a monotone model fitted on the device

@@ -59,3 +59,28 @@ def test_library_has_no_oracle_dependency():
     for src in glob.glob(os.path.join(ROOT, "turbosort_amd", "**", "*.*"), recursive=True):
         if src.endswith((".py", ".hip", ".h", ".cpp")):
             assert "oracle" not in open(src).read().lower().replace("oracle_", ""), src
+
+
+def test_segmented_variant_tables_are_consistent():
+    """Host-only queries (no GPU call): every size class has a default shape,
+    4- and 8-byte keys offer alternatives, 1- and 2-byte keys do not, and
+    8-byte keys with values stop at 8192 keys."""
+    from turbosort_amd import _native
+
+    L = _native.load()
+    for code in range(10):
+        width = [1, 2, 4, 8, 1, 2, 4, 8, 4, 8][code]
+        for vals in (0, 1):
+            counts = [L.ts_segmented_variants(code, vals, c) for c in range(9)]
+            top = 0 if (width == 8 and vals) else 1
+            assert all(n >= 1 for n in counts[:8]) and counts[8] >= top
+            if width <= 2:
+                assert max(counts) == 1
+            else:
+                assert max(counts) > 1
+            for c, n in enumerate(counts):
+                for v in range(n):
+                    assert L.ts_segmented_variant_name(code, vals, c, v)
+            cap = L.ts_segmented_lds_max_len(code, vals)
+            assert cap == (8192 if (width == 8 and vals) else 16384)
+    assert L.ts_segmented_variants(2, 1, 9) == 0 and L.ts_segmented_variants(2, 1, -1) == 0

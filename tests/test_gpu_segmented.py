@@ -193,3 +193,50 @@ def test_graph_capture_replays(gpu):
         torch.cuda.synchronize()
         assert np.array_equal(to_host(tk, np.uint32), want_k)
         assert np.array_equal(to_host(tv, np.uint32), want_v)
+
+
+@pytest.mark.parametrize("dtype,with_values", [("u32", True), ("u64", False), ("f32", True), ("i64", True),
+                                               ("u64", True)])
+def test_every_kernel_variant_is_exact(gpu, dtype, with_values):
+    """Every kernel shape the autotuner may pick sorts exactly."""
+    from turbosort_amd import autotune
+
+    tdt = {"u32": torch.uint32, "u64": torch.uint64, "f32": torch.float32, "i64": torch.int64}[dtype]
+    ctx = ts.Context(gpu)
+    g = rng(91)
+    for c, cap in enumerate(autotune.CLASS_CAPS):
+        names = autotune.variants(tdt, with_values, c)
+        if len(names) <= 1:
+            continue
+        lo = autotune.CLASS_CAPS[c - 1] + 1
+        lens = g.integers(lo, cap + 1, size=7)
+        off = offsets_of(list(lens))
+        n = int(off[-1])
+        keys = random_of(dtype, n, seed=c * 10 + len(dtype))
+        keys[: n // 3] = keys[g.integers(0, max(1, n // 3), size=n // 3)]  # ties
+        vals = np.arange(n, dtype=np.uint32) if with_values else None
+        want_k, want_v = cf.segmented_sort(keys, off, vals)
+        for v in range(len(names)):
+            autotune.set_variant(ctx, tdt, with_values, c, v)
+            tk, tv = to_dev(keys, gpu), None if vals is None else to_dev(vals, gpu)
+            ts.segmented_sort(tk, to_dev(off, gpu), tv, max_segment_len=cap, ctx=ctx)
+            torch.cuda.synchronize()
+            assert np.array_equal(bits(to_host(tk, keys.dtype)), bits(want_k)), (c, names[v])
+            if vals is not None:
+                assert np.array_equal(to_host(tv, np.uint32), want_v), (c, names[v])
+
+
+def test_autotuner_picks_a_variant_per_class(gpu):
+    from turbosort_amd import autotune
+
+    ctx = ts.Context(gpu)
+    table = autotune.tune_segmented(torch.uint32, True, keys_per_class=1 << 18, repeats=1, ctx=ctx)
+    assert set(table["choice"]) == {c for c in range(9) if len(autotune.variants(torch.uint32, True, c)) > 1}
+    for c, v in table["choice"].items():
+        assert table["times_ms"][c][v] == min(table["times_ms"][c])
+    off, keys, vals = cf.segmented_workload(7, 500)
+    tk, tv = to_dev(keys, gpu), to_dev(vals, gpu)
+    ts.segmented_sort(tk, to_dev(off, gpu), tv, max_segment_len=16384, ctx=ctx)
+    torch.cuda.synchronize()
+    want_k, want_v = cf.segmented_sort(keys, off, vals)
+    assert np.array_equal(to_host(tk, np.uint32), want_k) and np.array_equal(to_host(tv, np.uint32), want_v)

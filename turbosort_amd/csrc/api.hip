@@ -24,6 +24,8 @@ struct ts_ctx {
   size_t ws_bytes = 0;
   std::string err;
   ts::KernelClock clk;
+  // Segmented sort: kernel shape per (key width 1/2/4/8, values, size class).
+  uint8_t seg_variant[4][2][ts::kSegClasses] = {};
 };
 
 namespace {
@@ -187,6 +189,12 @@ ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "segment gather");
 }
 
+int width_index(int kb) { return kb == 1 ? 0 : kb == 2 ? 1 : kb == 4 ? 2 : 3; }
+
+void set_variants(const ts_ctx* c, int kb, bool vals, ts::SegJob& j) {
+  for (int k = 0; k < ts::kSegClasses; ++k) j.variant[k] = c->seg_variant[width_index(kb)][vals ? 1 : 0][k];
+}
+
 // Device arrays for sort_long_keys: room for `f` long segments per round.
 size_t long_keys_bytes(size_t f) {
   return 5 * up(f * 8) + up(f * 4) + 2 * up(2 * f * 8) + up((ts::kSegClasses + 2) * 4) +
@@ -293,6 +301,7 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
       gj.counts = d_counts;
       gj.lists = d_lists;
       gj.num_cus = c->num_cus;
+      set_variants(c, kb, false, gj);
       if (e == hipSuccess) e = ts::segmented_sort(gj, s, &c->clk);
       if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
       // The next round reuses d_starts / d_ends only after this sort (same stream).
@@ -495,6 +504,7 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
   j.counts = reinterpret_cast<uint32_t*>(c->ws);
   j.lists = reinterpret_cast<uint32_t*>(c->ws + count_bytes);
   j.num_cus = c->num_cus;
+  set_variants(c, kb, vals != nullptr, j);
   hipError_t e = ts::segmented_sort(j, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
   if (max_segment_len != 0 && max_segment_len <= ts::seg_lds_max_len(kb, vals != nullptr)) return TS_OK;
@@ -631,6 +641,7 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   sj.counts = reinterpret_cast<uint32_t*>(w + off[12]);
   sj.lists = reinterpret_cast<uint32_t*>(w + off[13]);
   sj.num_cus = c->num_cus;
+  set_variants(c, kb, false, sj);
   e = ts::segmented_sort(sj, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
 
@@ -684,6 +695,26 @@ ts_status ts_pwl_fit(ts_ctx* c, const float* u, const float* t, size_t T, float*
   DeviceScope ds(c->device);
   hipError_t e = ts::pwl_fit(u, t, uint32_t(T), theta, grad, loss2, iters, lr, static_cast<hipStream_t>(stream));
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "pwl_fit");
+}
+
+int ts_segmented_variants(ts_dtype dt, int with_values, int size_class) {
+  if (!valid_dtype(dt)) return 0;
+  return ts::seg_variants(ts::dtype_bytes(dt), with_values != 0, size_class);
+}
+
+const char* ts_segmented_variant_name(ts_dtype dt, int with_values, int size_class, int variant) {
+  if (!valid_dtype(dt)) return nullptr;
+  return ts::seg_variant_name(ts::dtype_bytes(dt), with_values != 0, size_class, variant);
+}
+
+ts_status ts_segmented_set_variant(ts_ctx* c, ts_dtype dt, int with_values, int size_class, int variant) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(dt)) return fail(c, TS_ERR_INVALID_ARG, "unknown dtype");
+  const int kb = ts::dtype_bytes(dt);
+  if (variant < 0 || variant >= ts::seg_variants(kb, with_values != 0, size_class))
+    return fail(c, TS_ERR_INVALID_ARG, "no such size class or variant");
+  c->seg_variant[width_index(kb)][with_values ? 1 : 0][size_class] = uint8_t(variant);
+  return TS_OK;
 }
 
 size_t ts_segmented_lds_max_len(ts_dtype dt, int with_values) {

@@ -139,8 +139,13 @@ struct SegJob {
   uint32_t* counts = nullptr;  // kSegClasses + 2 words
   uint32_t* lists = nullptr;   // kSegClasses * n_segments words
   int num_cus = 256;
+  uint8_t variant[kSegClasses] = {};  // kernel shape per size class (0: default)
 };
 uint32_t seg_lds_max_len(int key_bytes, bool vals);
+// Kernel shapes a size class can run (variant 0 is the default), and a name
+// for each.
+int seg_variants(int key_bytes, bool vals, int size_class);
+const char* seg_variant_name(int key_bytes, bool vals, int size_class, int variant);
 
 // Three-way split of long keys-only segments (segmented.hip): segment s is
 // [lo[s], lo[s] + len[s]), cut into chunks of split_chunk() keys numbered

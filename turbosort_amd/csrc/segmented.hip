@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <type_traits>
 
 #include "device.h"
@@ -701,21 +702,46 @@ struct ClassShape {
   bool bytes;  // byte digits (k_seg_radix8) instead of nibbles
 };
 
-template <class U, bool VALS> constexpr ClassShape class_shape(int c) {
+// Class c, variant v: variant 0 is the default; the others are the
+// alternatives an autotuner may pick (same capacity, another kernel shape).
+// Only 4- and 8-byte keys carry alternatives.
+template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0) {
   // 8-byte keys with values: 16384 keys would need 192 KB of LDS.
   constexpr bool wide = sizeof(U) == 8;
+  constexpr bool alt = sizeof(U) >= 4;
   switch (c) {
-    case 0: return {16, 0, 0, false};
-    case 1: return {32, 0, 0, false};
-    case 2: return {64, 0, 0, false};
-    case 3: return {0, 64, 4, false};
-    case 4: return {0, 64, 16, false};
-    case 5: return {0, 128, 16, false};
-    case 6: return {0, 256, 16, true};
-    case 7: return {0, 512, 16, true};
-    case 8: return wide && VALS ? ClassShape{0, 0, 0, false} : ClassShape{0, 1024, 16, true};
-    default: return {0, 0, 0, false};
+    case 0: return v == 0 ? ClassShape{16, 0, 0, false} : ClassShape{};
+    case 1: return v == 0 ? ClassShape{32, 0, 0, false} : ClassShape{};
+    case 2: return v == 0 ? ClassShape{64, 0, 0, false} : ClassShape{};
+    case 3: return v == 0 ? ClassShape{0, 64, 4, false} : ClassShape{};
+    case 4:
+      return v == 0 ? ClassShape{0, 64, 16, false} : (alt && v == 1 ? ClassShape{0, 256, 4, true} : ClassShape{});
+    case 5:
+      return v == 0 ? ClassShape{0, 128, 16, false} : (alt && v == 1 ? ClassShape{0, 256, 8, true} : ClassShape{});
+    case 6:
+      return v == 0 ? ClassShape{0, 256, 16, true} : (alt && v == 1 ? ClassShape{0, 256, 16, false} : ClassShape{});
+    case 7:
+      return v == 0   ? ClassShape{0, 512, 16, true}
+             : !alt   ? ClassShape{}
+             : v == 1 ? ClassShape{0, 512, 16, false}
+             : v == 2 ? ClassShape{0, 1024, 8, true}
+                      : ClassShape{};
+    case 8:
+      if (wide && VALS) return ClassShape{};
+      return v == 0 ? ClassShape{0, 1024, 16, true} : (alt && v == 1 ? ClassShape{0, 512, 32, false} : ClassShape{});
+    default: return ClassShape{};
   }
+}
+
+template <class U, bool VALS> constexpr bool class_exists(int c, int v) {
+  const ClassShape s = class_shape<U, VALS>(c, v);
+  return s.bitonic_w > 0 || s.threads > 0;
+}
+
+template <class U, bool VALS> constexpr int class_variants(int c) {
+  int n = 0;
+  while (n < 4 && class_exists<U, VALS>(c, n)) ++n;
+  return n;
 }
 
 template <class U, bool VALS> constexpr uint32_t class_cap(int c) {
@@ -723,9 +749,9 @@ template <class U, bool VALS> constexpr uint32_t class_cap(int c) {
   return s.bitonic_w ? uint32_t(s.bitonic_w) : uint32_t(s.threads * s.ipt);
 }
 
-template <class U, bool VALS, int C>
-void launch_class(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk) {
-  constexpr ClassShape sh = class_shape<U, VALS>(C);
+template <class U, bool VALS, int C, int V>
+void launch_shape(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk) {
+  constexpr ClassShape sh = class_shape<U, VALS>(C, V);
   if constexpr (sh.bitonic_w > 0) {
     if (clk) clk->begin(s, "k_seg_bitonic", 0);
     hipLaunchKernelGGL((k_seg_bitonic<U, sh.bitonic_w, VALS>), dim3(num_cus * 8), dim3(256), 0, s, a, C);
@@ -751,6 +777,39 @@ void launch_class(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk
   }
 }
 
+template <class U, bool VALS, int C>
+void launch_class(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk, int v) {
+  if (v == 1) {
+    if constexpr (class_exists<U, VALS>(C, 1)) return launch_shape<U, VALS, C, 1>(a, num_cus, s, clk);
+  } else if (v == 2) {
+    if constexpr (class_exists<U, VALS>(C, 2)) return launch_shape<U, VALS, C, 2>(a, num_cus, s, clk);
+  }
+  launch_shape<U, VALS, C, 0>(a, num_cus, s, clk);
+}
+
+template <class U, bool VALS> const char* variant_name(int c, int v) {
+  static char buf[kSegClasses][4][48];
+  if (c < 0 || c >= kSegDevice || v < 0 || v >= class_variants<U, VALS>(c)) return nullptr;
+  ClassShape sh{};
+  switch (c) {
+    case 0: sh = class_shape<U, VALS>(0, v); break;
+    case 1: sh = class_shape<U, VALS>(1, v); break;
+    case 2: sh = class_shape<U, VALS>(2, v); break;
+    case 3: sh = class_shape<U, VALS>(3, v); break;
+    case 4: sh = class_shape<U, VALS>(4, v); break;
+    case 5: sh = class_shape<U, VALS>(5, v); break;
+    case 6: sh = class_shape<U, VALS>(6, v); break;
+    case 7: sh = class_shape<U, VALS>(7, v); break;
+    default: sh = class_shape<U, VALS>(8, v); break;
+  }
+  if (sh.bitonic_w)
+    std::snprintf(buf[c][v], sizeof buf[c][v], "bitonic, %d lanes per segment", sh.bitonic_w);
+  else
+    std::snprintf(buf[c][v], sizeof buf[c][v], "%s digits, %d lanes x %d keys", sh.bytes ? "byte" : "4-bit",
+                  sh.threads, sh.ipt);
+  return buf[c][v];
+}
+
 template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStream_t s, KernelClock* clk) {
   SegArgs a;
   a.keys = j.keys;
@@ -770,15 +829,15 @@ template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStrea
   hipLaunchKernelGGL(k_seg_classify, dim3(cgrid), dim3(kClassifyThreads), 0, s, a);
   if (clk) clk->end(s);
   // Longest segments first, so the short classes fill in behind them.
-  launch_class<U, VALS, 8>(a, j.num_cus, s, clk);
-  launch_class<U, VALS, 7>(a, j.num_cus, s, clk);
-  launch_class<U, VALS, 6>(a, j.num_cus, s, clk);
-  launch_class<U, VALS, 5>(a, j.num_cus, s, clk);
-  launch_class<U, VALS, 4>(a, j.num_cus, s, clk);
-  launch_class<U, VALS, 3>(a, j.num_cus, s, clk);
-  launch_class<U, VALS, 2>(a, j.num_cus, s, clk);
-  launch_class<U, VALS, 1>(a, j.num_cus, s, clk);
-  launch_class<U, VALS, 0>(a, j.num_cus, s, clk);
+  launch_class<U, VALS, 8>(a, j.num_cus, s, clk, j.variant[8]);
+  launch_class<U, VALS, 7>(a, j.num_cus, s, clk, j.variant[7]);
+  launch_class<U, VALS, 6>(a, j.num_cus, s, clk, j.variant[6]);
+  launch_class<U, VALS, 5>(a, j.num_cus, s, clk, j.variant[5]);
+  launch_class<U, VALS, 4>(a, j.num_cus, s, clk, j.variant[4]);
+  launch_class<U, VALS, 3>(a, j.num_cus, s, clk, j.variant[3]);
+  launch_class<U, VALS, 2>(a, j.num_cus, s, clk, j.variant[2]);
+  launch_class<U, VALS, 1>(a, j.num_cus, s, clk, j.variant[1]);
+  launch_class<U, VALS, 0>(a, j.num_cus, s, clk, j.variant[0]);
   return hipGetLastError();
 }
 
@@ -797,6 +856,44 @@ uint32_t seg_lds_max_len(int key_bytes, bool vals) {
     case 4: return vals ? max_cap<uint32_t, true>() : max_cap<uint32_t, false>();
     case 8: return vals ? max_cap<uint64_t, true>() : max_cap<uint64_t, false>();
     default: return 0;
+  }
+}
+
+int seg_variants(int key_bytes, bool vals, int size_class) {
+  if (size_class < 0 || size_class >= kSegDevice) return 0;
+  auto count = [&](auto u, auto v) {
+    using U = decltype(u);
+    constexpr bool V = decltype(v)::value;
+    switch (size_class) {
+      case 0: return class_variants<U, V>(0);
+      case 1: return class_variants<U, V>(1);
+      case 2: return class_variants<U, V>(2);
+      case 3: return class_variants<U, V>(3);
+      case 4: return class_variants<U, V>(4);
+      case 5: return class_variants<U, V>(5);
+      case 6: return class_variants<U, V>(6);
+      case 7: return class_variants<U, V>(7);
+      default: return class_variants<U, V>(8);
+    }
+  };
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  switch (key_bytes) {
+    case 1: return vals ? count(uint8_t(), T_()) : count(uint8_t(), F_());
+    case 2: return vals ? count(uint16_t(), T_()) : count(uint16_t(), F_());
+    case 4: return vals ? count(uint32_t(), T_()) : count(uint32_t(), F_());
+    case 8: return vals ? count(uint64_t(), T_()) : count(uint64_t(), F_());
+    default: return 0;
+  }
+}
+
+const char* seg_variant_name(int key_bytes, bool vals, int size_class, int variant) {
+  switch (key_bytes) {
+    case 1: return vals ? variant_name<uint8_t, true>(size_class, variant) : variant_name<uint8_t, false>(size_class, variant);
+    case 2: return vals ? variant_name<uint16_t, true>(size_class, variant) : variant_name<uint16_t, false>(size_class, variant);
+    case 4: return vals ? variant_name<uint32_t, true>(size_class, variant) : variant_name<uint32_t, false>(size_class, variant);
+    case 8: return vals ? variant_name<uint64_t, true>(size_class, variant) : variant_name<uint64_t, false>(size_class, variant);
+    default: return nullptr;
   }
 }
 
