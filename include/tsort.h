@@ -193,6 +193,17 @@ ts_status ts_checksum(ts_ctx* ctx, ts_dtype dtype, const void* d_data, size_t n,
 ts_status ts_ctx_set_profiling(ts_ctx* ctx, int enable);
 int ts_ctx_kernel_times(const ts_ctx* ctx, const char** names, float* ms, double* bytes, int cap);
 
+/* How the scatter passes rank a wave's keys by digit (no counterpart in the
+ * reference, whose scatter is sequential: scatter.rs:26-40).  Both give the
+ * same stable order.  TS_RANK_ATOMIC uses LDS atomic adds and relies on the
+ * adds of one wave instruction to one address taking effect in lane order;
+ * ts_ctx_create checks that once per device and selects it when it holds,
+ * else TS_RANK_BALLOT (eight wave ballots per key).  Setting TS_RANK_ATOMIC
+ * on a device that failed the check returns TS_ERR_INVALID_ARG. */
+enum { TS_RANK_BALLOT = 0, TS_RANK_ATOMIC = 1 };
+ts_status ts_ctx_set_ranking(ts_ctx* ctx, int method);
+int ts_ctx_ranking(const ts_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

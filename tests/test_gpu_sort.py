@@ -15,7 +15,7 @@ from conftest import kat_array, kats
 from gpu_util import bits, to_dev, to_host
 from oracle import closed_form as cf
 from oracle import native as nv
-from shapes import adversarial_u32, float_specials_f32, random_of, structured_u64
+from shapes import adversarial_u32, float_specials_f32, random_of, rng, structured_u64
 
 pytestmark = pytest.mark.gpu
 
@@ -156,3 +156,34 @@ def test_sorts_on_a_side_stream(gpu):
         ts.sort(t)
     torch.cuda.current_stream(gpu).wait_stream(s)
     assert np.array_equal(to_host(t, np.uint32), np.sort(x))
+
+
+def test_both_ranking_methods(gpu):
+    """The scatter passes rank a wave's keys by LDS atomic adds where the
+    device passed the context's lane-order check (MI355X does), else by
+    ballots.  Both must give the oracle's output: keys, and the stable order of
+    equal keys (a long segment of the segmented sort runs the by-key passes
+    with positions as values)."""
+    ctx = ts.Context(gpu)
+    assert ctx.ranking == "atomic"
+    g = rng(91)
+    skew = np.where(g.random(600_000) < 0.9, np.uint32(12345), random_of("u32", 600_000, seed=92))
+    dup = (random_of("u32", 400_000, seed=93) % 1000).astype(np.uint32)
+    for method in ("ballot", "atomic"):
+        ctx.set_ranking(method)
+        assert ctx.ranking == method
+        for name, n in (("u32", 1_000_003), ("u64", 700_001), ("f32", 65_537), ("i16", 300_000), ("f64", 250_000)):
+            x = random_of(name, n, seed=n)
+            t = to_dev(x, gpu)
+            ts.sort(t, ctx=ctx)
+            assert np.array_equal(bits(to_host(t, x.dtype)), bits(cf.sort(x))), (method, name, n)
+        t = to_dev(skew, gpu)
+        ts.sort(t, ctx=ctx)
+        assert np.array_equal(to_host(t, np.uint32), np.sort(skew)), method
+        k = to_dev(dup, gpu)
+        v = torch.arange(dup.size, dtype=torch.int32, device=gpu)
+        off = torch.tensor([0, dup.size], dtype=torch.int64, device=gpu)
+        ts.segmented_sort(k, off, v, ctx=ctx)
+        want = np.argsort(dup, kind="stable").astype(np.uint32)
+        assert np.array_equal(v.cpu().numpy().astype(np.uint32), want), method
+        assert np.array_equal(to_host(k, np.uint32), dup[want]), method

@@ -104,6 +104,18 @@ class Context:
     def set_profiling(self, on: bool) -> None:
         self.check(self._lib.ts_ctx_set_profiling(self._h, 1 if on else 0))
 
+    RANKINGS = ("ballot", "atomic")  # TS_RANK_BALLOT, TS_RANK_ATOMIC
+
+    @property
+    def ranking(self) -> str:
+        """How the scatter passes rank keys within a wave: "atomic" (LDS
+        atomic adds, chosen where this device passed the lane-order check) or
+        "ballot"."""
+        return self.RANKINGS[self._lib.ts_ctx_ranking(self._h)]
+
+    def set_ranking(self, method: str) -> None:
+        self.check(self._lib.ts_ctx_set_ranking(self._h, self.RANKINGS.index(method)))
+
     def kernel_times(self):
         """[(kernel name, ms, algorithmic bytes)] of the last call, from HIP events."""
         cap = 64

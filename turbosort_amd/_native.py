@@ -20,6 +20,7 @@ EXPORTS = [
     "ts_sbk_last_error", "ts_segmented_sort_pairs", "ts_segmented_lds_max_len",
     "ts_learned_cfg_default", "ts_learned_sort", "ts_pwl_fit",
     "ts_segmented_variants", "ts_segmented_variant_name", "ts_segmented_set_variant",
+    "ts_ctx_set_ranking", "ts_ctx_ranking",
 ]
 
 
@@ -101,6 +102,10 @@ def load() -> ctypes.CDLL:
     L.ts_checksum.restype = i32
     L.ts_ctx_set_profiling.argtypes = [vp, i32]
     L.ts_ctx_set_profiling.restype = i32
+    L.ts_ctx_set_ranking.argtypes = [vp, i32]
+    L.ts_ctx_set_ranking.restype = i32
+    L.ts_ctx_ranking.argtypes = [vp]
+    L.ts_ctx_ranking.restype = i32
     L.ts_ctx_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_double), i32]
     L.ts_ctx_kernel_times.restype = i32

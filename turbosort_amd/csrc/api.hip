@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,10 @@ struct ts_ctx {
   ts::KernelClock clk;
   // Segmented sort: kernel shape per (key width 1/2/4/8, values, size class).
   uint8_t seg_variant[4][2][ts::kSegClasses] = {};
+  // Whether this device applies a wave's LDS atomic adds to one address in
+  // lane order (rank_order_check), and whether the sorts use that ranking.
+  bool atomic_rank_ok = false;
+  bool atomic_rank = false;
 };
 
 namespace {
@@ -117,6 +122,7 @@ ts_status sort_impl(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, hip
   j.keys_a = data;
   j.keys_b = buf ? buf : p;
   j.num_cus = c->num_cus;
+  j.atomic_rank = c->atomic_rank;
   hipError_t e = ts::radix_sort(j, s, &c->clk);
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort");
 }
@@ -137,6 +143,7 @@ ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint
     j.keys_a = keys;
     j.keys_b = c->ws + up(sizeof(ts::SortMeta)) + up(ts::status_bytes(m, kb));
     j.num_cus = c->num_cus;
+    j.atomic_rank = c->atomic_rank;
     hipError_t e = ts::radix_sort(j, s, &c->clk);
     return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort");
   }
@@ -177,6 +184,7 @@ ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint
     gbuf = p;
     j.final_vals = order;
     j.num_cus = c->num_cus;
+    j.atomic_rank = c->atomic_rank;
     e = ts::radix_sort(j, s, &c->clk);
     if (e != hipSuccess) return hip_fail(c, e, "radix_sort");
   }
@@ -335,19 +343,61 @@ const char* ts_status_string(ts_status s) {
   return "unknown status";
 }
 
+namespace {
+
+// Runs rank_order_check once per device (first context on it) and keeps the
+// answer: 1 when every atomic rank matched its ballot rank, 0 otherwise.
+hipError_t atomic_rank_ok(int device, bool* ok) {
+  static std::mutex mu;
+  static std::vector<int8_t> known;
+  std::lock_guard<std::mutex> lock(mu);
+  if (size_t(device) >= known.size()) known.resize(size_t(device) + 1, -1);
+  if (known[device] < 0) {
+    DeviceScope ds(device);
+    uint32_t* bad = nullptr;
+    hipError_t e = hipMalloc(&bad, sizeof(uint32_t));
+    if (e != hipSuccess) return e;
+    uint32_t h = 1;
+    e = hipMemset(bad, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = ts::rank_order_check(bad, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(&h, bad, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    (void)hipFree(bad);
+    if (e != hipSuccess) return e;
+    known[device] = h == 0 ? 1 : 0;
+  }
+  *ok = known[device] == 1;
+  return hipSuccess;
+}
+
+}  // namespace
+
 ts_status ts_ctx_create(ts_ctx** out, int device) {
   if (!out) return TS_ERR_INVALID_ARG;
   *out = nullptr;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return TS_ERR_NO_DEVICE;
+  bool ok = false;
+  if (atomic_rank_ok(device, &ok) != hipSuccess) return TS_ERR_HIP;
   ts_ctx* c = new ts_ctx();
   c->device = device;
+  c->atomic_rank_ok = ok;
+  c->atomic_rank = ok;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     c->num_cus = cus;
   *out = c;
   return TS_OK;
 }
+
+ts_status ts_ctx_set_ranking(ts_ctx* c, int method) {
+  if (!c || (method != TS_RANK_BALLOT && method != TS_RANK_ATOMIC)) return TS_ERR_INVALID_ARG;
+  if (method == TS_RANK_ATOMIC && !c->atomic_rank_ok)
+    return fail(c, TS_ERR_INVALID_ARG, "atomic ranking: this device's LDS atomics failed the lane-order check");
+  c->atomic_rank = method == TS_RANK_ATOMIC;
+  return TS_OK;
+}
+
+int ts_ctx_ranking(const ts_ctx* c) { return c && c->atomic_rank ? TS_RANK_ATOMIC : TS_RANK_BALLOT; }
 
 void ts_ctx_destroy(ts_ctx* c) {
   if (!c) return;
@@ -443,6 +493,7 @@ ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t
     p += up(n * 4);
     j.final_vals = order;
     j.num_cus = c->num_cus;
+    j.atomic_rank = c->atomic_rank;
     gbuf = p;
     e = ts::radix_sort(j, s, &c->clk);
     if (e != hipSuccess) return hip_fail(c, e, "radix_sort");
@@ -628,6 +679,7 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   j.meta = reinterpret_cast<ts::SortMeta*>(w + off[0]);
   j.status = w + off[1];
   j.num_cus = c->num_cus;
+  j.atomic_rank = c->atomic_rank;
   hipError_t e = ts::learned_partition(j, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "learned_partition");
 

@@ -118,6 +118,26 @@ __device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[(
   }
 }
 
+// The same ranks as rank_in_wave from LDS atomic adds on the wave's digit
+// counters: each add returns the count before it.  The adds of one wave
+// instruction to one counter take effect in lane order (a property of the
+// LDS that the context checks once per device before using this ranking),
+// and a wave's instructions in program order, so ranks follow the key order
+// j * 64 + lane, as stability needs.  Lanes past the tile's end add nothing.
+template <class U, int IPT, bool FULL>
+__device__ __forceinline__ void rank_in_wave_atomic(const U (&k)[IPT], uint32_t (&rk)[(IPT + 1) / 2], unsigned shift,
+                                                    uint32_t* whist, uint32_t lb, uint32_t valid) {
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    uint32_t r = 0;
+    if (FULL || lb + uint32_t(j) * 64 < valid) r = atomicAdd(&whist[digit_of(k[j], shift)], 1u);
+    if (j & 1)
+      rk[j >> 1] |= r << 16;
+    else
+      rk[j >> 1] = r;
+  }
+}
+
 // Forward key transform of N keys under one (wave-uniform) mode test.
 template <class U, int N> __device__ __forceinline__ void to_keys(U (&k)[N], int mode) {
   if (mode == kSigned) {

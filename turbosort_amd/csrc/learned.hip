@@ -446,6 +446,7 @@ template <class U> hipError_t run_learned(const LearnedJob& j, hipStream_t s, Ke
   r.meta = j.meta;
   r.status = j.status;
   r.num_cus = j.num_cus;
+  r.atomic_rank = j.atomic_rank;
   hipError_t e = j.sample > kSmallMax ? radix_sort(r, s, clk)
                                       : small_sort(8, kIdentity, j.sample_a, j.sample, nullptr, 0, 0, nullptr, s, clk);
   if (e != hipSuccess) return e;
@@ -478,6 +479,7 @@ template <class U> hipError_t run_learned(const LearnedJob& j, hipStream_t s, Ke
   p.meta = j.meta;
   p.status = j.status;
   p.num_cus = j.num_cus;
+  p.atomic_rank = j.atomic_rank;
   p.model = j.model;
   e = radix_sort(p, s, clk);
   if (e != hipSuccess) return e;

@@ -4,6 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "keys.h"
 
 namespace ts {
@@ -31,57 +34,77 @@ struct KernelClock {
   }
 };
 
-// Tile geometry of the one-sweep scatter per key width: `threads` lanes per
-// workgroup (a multiple of 256; the first 256 lanes also carry one digit
-// each through the per-digit steps) and `ipt` keys per lane.  Measured on
-// MI355X at 2^28 keys: 4-byte keys are VALU-bound and prefer 12288-key tiles
-// at full occupancy; 8-byte keys are bound by the scatter's write runs and
-// prefer 8192-key tiles (256-byte average runs per digit).
-#ifndef TS_SWEEP_THREADS
-#define TS_SWEEP_THREADS 512
+// Tile geometry of the one-sweep scatter: `threads` lanes per workgroup (a
+// multiple of 256; the first 256 lanes also carry one digit each through the
+// per-digit steps) and `ipt` keys per lane, per key width (4-byte and
+// narrower / 8-byte) and kind: keys only, keys with values (both staged in
+// LDS), keys ordered by learned buckets (whose staged digits and model knots
+// also take LDS).  Measured on MI355X at 2^28 keys with LDS-atomic ranking:
+// the passes are bound by memory latency and the scattered write runs, and
+// larger tiles (longer runs per digit, fewer look-back steps) win down to one
+// workgroup per CU.
+#ifndef TS_SWEEP_KEYS_NARROW
+#define TS_SWEEP_KEYS_NARROW 1024, 32
 #endif
-#ifndef TS_SWEEP_IPT_NARROW
-#define TS_SWEEP_IPT_NARROW 24
+#ifndef TS_SWEEP_KEYS_WIDE
+#define TS_SWEEP_KEYS_WIDE 512, 36
 #endif
-#ifndef TS_SWEEP_IPT_WIDE
-#define TS_SWEEP_IPT_WIDE 16
+#ifndef TS_SWEEP_VALS_NARROW
+#define TS_SWEEP_VALS_NARROW 512, 32
+#endif
+#ifndef TS_SWEEP_VALS_WIDE
+#define TS_SWEEP_VALS_WIDE 512, 16
+#endif
+#ifndef TS_SWEEP_LEARNED_NARROW
+#define TS_SWEEP_LEARNED_NARROW 512, 32
+#endif
+#ifndef TS_SWEEP_LEARNED_WIDE
+#define TS_SWEEP_LEARNED_WIDE 512, 14
 #endif
 // Predecessor tiles one look-back step reads at once.
 #ifndef TS_LOOKBACK_WINDOW
 #define TS_LOOKBACK_WINDOW 4
 #endif
-// The learned partition's 8-byte tiles are smaller: the staged digits and the
-// model's knots also live in LDS, and two workgroups must still fit a CU.
-#ifndef TS_SWEEP_IPT_WIDE_LEARNED
-#define TS_SWEEP_IPT_WIDE_LEARNED 14
-#endif
-template <int KB, bool LEARNED = false> struct SweepShape {
-  static constexpr int threads = TS_SWEEP_THREADS;
-  static constexpr int ipt = KB >= 8 ? (LEARNED ? TS_SWEEP_IPT_WIDE_LEARNED : TS_SWEEP_IPT_WIDE) : TS_SWEEP_IPT_NARROW;
-  static constexpr int tile = threads * ipt;
+template <int THREADS, int IPT> struct ShapeOf {
+  static constexpr int threads = THREADS;
+  static constexpr int ipt = IPT;
+};
+template <int KB, bool LEARNED = false, bool VALS = false>
+struct SweepShape
+    : std::conditional_t<
+          (KB >= 8),
+          std::conditional_t<LEARNED, ShapeOf<TS_SWEEP_LEARNED_WIDE>,
+                             std::conditional_t<VALS, ShapeOf<TS_SWEEP_VALS_WIDE>, ShapeOf<TS_SWEEP_KEYS_WIDE>>>,
+          std::conditional_t<LEARNED, ShapeOf<TS_SWEEP_LEARNED_NARROW>,
+                             std::conditional_t<VALS, ShapeOf<TS_SWEEP_VALS_NARROW>, ShapeOf<TS_SWEEP_KEYS_NARROW>>>> {
+  static_assert(SweepShape::threads % 256 == 0, "the first 256 lanes carry one digit each");
+  static constexpr int tile = SweepShape::threads * SweepShape::ipt;
 };
 
-inline size_t sweep_tile(int key_bytes, bool learned = false) {
-  if (key_bytes < 8) return size_t(SweepShape<4>::tile);
-  return learned ? size_t(SweepShape<8, true>::tile) : size_t(SweepShape<8>::tile);
+inline size_t sweep_tile(int key_bytes, bool learned = false, bool vals = false) {
+  if (key_bytes < 8)
+    return size_t(learned ? SweepShape<4, true>::tile : vals ? SweepShape<4, false, true>::tile : SweepShape<4>::tile);
+  return size_t(learned ? SweepShape<8, true>::tile : vals ? SweepShape<8, false, true>::tile : SweepShape<8>::tile);
 }
 // Upper bound on the tiles of one pass: every segment may end in a partial tile.
-inline uint32_t sweep_tiles(size_t n, int key_bytes, bool learned = false) {
-  const size_t tile = sweep_tile(key_bytes, learned);
+inline uint32_t sweep_tiles(size_t n, int key_bytes, bool learned = false, bool vals = false) {
+  const size_t tile = sweep_tile(key_bytes, learned, vals);
   return uint32_t((n + tile - 1) / tile + kSegments);
 }
 // Length of the first pass's position segments (a whole number of tiles).
-inline size_t first_segment(size_t n, int key_bytes, bool learned = false) {
-  const size_t tile = sweep_tile(key_bytes, learned);
+inline size_t first_segment(size_t n, int key_bytes, bool learned = false, bool vals = false) {
+  const size_t tile = sweep_tile(key_bytes, learned, vals);
   const size_t per = (n + kSegments - 1) / kSegments;
   return ((per + tile - 1) / tile) * tile;
 }
 
 // Look-back words are 32-bit (2 flag bits, 30 value bits) while every count
-// fits in 30 bits, else 64-bit.
+// fits in 30 bits, else 64-bit.  Sized for the larger tile count of the two
+// tile shapes (with and without values).
 inline bool wide_status(size_t n) { return n >= (size_t(1) << 30); }
 inline size_t status_bytes(size_t n, int key_bytes, bool learned = false) {
-  return size_t(2) * sweep_tiles(n, key_bytes, learned) * 256 * (wide_status(n) ? 8 : 4);
+  const uint32_t tiles = std::max(sweep_tiles(n, key_bytes, learned, false), sweep_tiles(n, key_bytes, learned, true));
+  return size_t(2) * tiles * 256 * (wide_status(n) ? 8 : 4);
 }
 
 // One LSD sort on the device.  Keys start in keys_a (plain sort) or are read
@@ -104,9 +127,16 @@ struct RadixJob {
   // bucket (up to 24 bits) instead of by the key (keys only; 4- and 8-byte
   // keys).
   const LearnedModel* model = nullptr;
+  // Rank keys within a wave by LDS atomic adds instead of ballots; only where
+  // rank_order_check found the adds applied in lane order.
+  bool atomic_rank = false;
 };
 
 hipError_t radix_sort(const RadixJob& job, hipStream_t s, KernelClock* clk);
+
+// Adds to *bad (device memory, zeroed by the caller) the number of keys whose
+// LDS-atomic rank differs from their ballot rank over a set of digit patterns.
+hipError_t rank_order_check(uint32_t* bad, hipStream_t s);
 
 // Small-input tier: one workgroup sorts n <= kSmallMax keys in LDS with a
 // bitonic network.  With order != nullptr it sorts (key, position) pairs and
@@ -200,6 +230,7 @@ struct LearnedJob {
   SortMeta* meta = nullptr;
   void* status = nullptr;  // status_bytes(n, key_bytes)
   int num_cus = 256;
+  bool atomic_rank = false;  // as RadixJob::atomic_rank
 };
 hipError_t learned_partition(const LearnedJob& job, hipStream_t s, KernelClock* clk);
 

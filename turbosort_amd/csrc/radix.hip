@@ -31,8 +31,6 @@ template <class U> __device__ __forceinline__ U first_lane(U x) {
   }
 }
 
-// Emits nothing; the compiler can no longer see x's value, so values it
-// derived from x earlier are recomputed rather than kept live.
 // ------------------------------------------------------------------ init
 __global__ void k_init(SortMeta* m, int P) {
   for (unsigned i = threadIdx.x; i < unsigned(P) * kSegments * 256; i += blockDim.x) (&m->pair[0][0][0])[i] = 0;
@@ -289,6 +287,7 @@ template <class U> struct SweepArgs {
   uint32_t* error;  // set when a look-back wait exceeds its bound
   uint32_t num_tiles;
   const LearnedModel* model;  // learned partition: digits of the key's bucket
+  int atomic_rank;            // rank with LDS atomics (rank_in_wave_atomic), else ballots
 };
 
 // About two seconds of s_sleep(1) polling.
@@ -312,8 +311,8 @@ constexpr uint32_t kSpinLimit = 1u << 26;
 // order to final_vals.
 // LDS of one scatter tile, declared once in the kernel.
 template <class U, bool VALS, int IPT, bool LEARNED = false> struct SweepLds {
-  static constexpr int NW = SweepShape<sizeof(U), LEARNED>::threads / 64;
-  static constexpr int TILE = SweepShape<sizeof(U), LEARNED>::threads * IPT;
+  static constexpr int NW = SweepShape<sizeof(U), LEARNED, VALS>::threads / 64;
+  static constexpr int TILE = SweepShape<sizeof(U), LEARNED, VALS>::threads * IPT;
   uint32_t whist[NW][256];  // per-wave digit counts, then staging bases
   U keys[TILE];
   uint32_t vals[VALS ? TILE : 1];
@@ -329,13 +328,13 @@ template <class U, bool VALS, int IPT, bool LEARNED = false> struct SweepLds {
 // counts, rank, look back, stage, store.  FULL tiles (all but each segment's
 // last) run without any per-key bounds test.
 template <class U, bool VALS, class S, bool FULL, bool LEARNED>
-__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED>::ipt, LEARNED>& L,
+__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS>::ipt, LEARNED>& L,
                                            const SweepArgs<U>& a,
                                            S* __restrict__ status, int pass, unsigned shift, bool first, bool last,
                                            size_t tile_base, uint32_t valid, size_t first_slot, size_t slot,
                                            bool first_round, unsigned long long digit_base) {
-  constexpr int IPT = SweepShape<sizeof(U), LEARNED>::ipt;
-  constexpr int THREADS = SweepShape<sizeof(U), LEARNED>::threads;
+  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
   constexpr int VB = Flags<S>::kBits;
@@ -386,26 +385,35 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
       return digit_of(k[j], shift);
   };
 
-  // Publish the tile's digit counts before ranking, so that later tiles
-  // rarely find this one unpublished.
   S* my = row + slot * 256 + tid;
-#pragma unroll
-  for (int j = 0; j < IPT; ++j)
-    if (in_tile(j)) atomicAdd(&L.tcnt[dig(j)], 1u);
-  __syncthreads();
-  if (dl) __hip_atomic_store(my, S(L.tcnt[tid]) | (first_round ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  // Emit nothing, but make the keys opaque so that the digits computed above
-  // are recomputed (one v_bfe each) instead of being held in IPT registers.
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) opaque(k[j]);
-
   uint32_t rk[(IPT + 1) / 2];
-  if constexpr (LEARNED)
-    rank_in_wave<uint32_t, IPT, FULL>(bk, rk, shift, L.whist[w], lb, valid);
-  else
-    rank_in_wave<U, IPT, FULL>(k, rk, shift, L.whist[w], lb, valid);
+  if (a.atomic_rank) {
+    // Ranking takes a few instructions per key: the counts are published
+    // right after it.
+    if constexpr (LEARNED)
+      rank_in_wave_atomic<uint32_t, IPT, FULL>(bk, rk, shift, L.whist[w], lb, valid);
+    else
+      rank_in_wave_atomic<U, IPT, FULL>(k, rk, shift, L.whist[w], lb, valid);
+  } else {
+    // Publish the tile's digit counts before ranking, so that later tiles
+    // rarely find this one unpublished.
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) opaque(k[j]);  // again: recompute digits when staging
+    for (int j = 0; j < IPT; ++j)
+      if (in_tile(j)) atomicAdd(&L.tcnt[dig(j)], 1u);
+    __syncthreads();
+    if (dl)
+      __hip_atomic_store(my, S(L.tcnt[tid]) | (first_round ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Emit nothing, but make the keys opaque so that the digits computed above
+    // are recomputed (one v_bfe each) instead of being held in IPT registers.
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) opaque(k[j]);
+    if constexpr (LEARNED)
+      rank_in_wave<uint32_t, IPT, FULL>(bk, rk, shift, L.whist[w], lb, valid);
+    else
+      rank_in_wave<U, IPT, FULL>(k, rk, shift, L.whist[w], lb, valid);
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) opaque(k[j]);  // recompute digits when staging
   __syncthreads();
 
   // Per digit: exclusive offsets of the waves, and the tile's count.
@@ -416,6 +424,8 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
     wexcl[ww] = cnt;
     if (dl) cnt += L.whist[ww][tid];
   }
+  if (a.atomic_rank && dl)
+    __hip_atomic_store(my, S(cnt) | (first_round ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t excl_in_tile = block256_exclusive_add(cnt, L.wave);
   if (dl) {
 #pragma unroll
@@ -532,10 +542,10 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
 // pass applies the inverse key transform, and in a by-key sort writes only the
 // order to final_vals.
 template <class U, bool VALS, class S, bool LEARNED = false>
-__global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED>::threads)) void k_sweep(SweepArgs<U> a,
+__global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS>::threads)) void k_sweep(SweepArgs<U> a,
                                                                                   S* __restrict__ status, int pass) {
-  constexpr int IPT = SweepShape<sizeof(U), LEARNED>::ipt;
-  constexpr int THREADS = SweepShape<sizeof(U), LEARNED>::threads;
+  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
   __shared__ SweepLds<U, VALS, IPT, LEARNED> L;
@@ -603,7 +613,8 @@ __global__ void k_iota_if_dead(const SortMeta* m, uint32_t* out, size_t n) {
 }
 
 template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk) {
-  const uint32_t tiles = sweep_tiles(j.n, sizeof(U), LEARNED);
+  const bool vals = j.final_vals != nullptr;
+  const uint32_t tiles = sweep_tiles(j.n, sizeof(U), LEARNED, vals);
   SweepArgs<U> a;
   a.keys_a = static_cast<U*>(j.keys_a);
   a.keys_b = static_cast<U*>(j.keys_b);
@@ -619,7 +630,7 @@ template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& 
   a.error = &j.meta->error;
   a.num_tiles = tiles;
   a.model = j.model;
-  const bool vals = j.final_vals != nullptr;
+  a.atomic_rank = j.atomic_rank ? 1 : 0;
   const double bytes = double(j.n) * (2.0 * sizeof(U) + (vals ? 8.0 : 0.0));
   for (int p = 0; p < DigitSource<U, LEARNED>::P; ++p) {
     if (clk) clk->begin(s, "k_sweep", bytes);
@@ -627,7 +638,8 @@ template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& 
       hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U), true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
     else if (vals)
-      hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
+      hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U), false, true>::threads), 0, s, a,
+                         static_cast<S*>(j.status), p);
     else
       hipLaunchKernelGGL((k_sweep<U, false, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
     if (clk) clk->end(s);
@@ -641,12 +653,13 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, s, j.meta, P);
   // Histogram: blocks_per_seg workgroups per position segment, each over a
   // contiguous chunk of it.
-  const size_t seg0 = first_segment(j.n, KB, LEARNED);
+  const bool vals = j.final_vals != nullptr;
+  const size_t seg0 = first_segment(j.n, KB, LEARNED, vals);
   const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * HistUnroll<U>::blocks_per_cu / kSegments));
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
   size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
   per_block = std::max(step, ((per_block + step - 1) / step) * step);
-  const uint32_t tiles = sweep_tiles(j.n, KB, LEARNED);
+  const uint32_t tiles = sweep_tiles(j.n, KB, LEARNED, vals);
   const size_t zero_words = size_t(tiles) * 256 * (wide_status(j.n) ? 2 : 1);
   const dim3 hgrid(blocks_per_seg * kSegments);
   if (clk) clk->begin(s, "k_hist", double(j.n) * (j.records ? double(j.stride) + KB : KB));
@@ -663,7 +676,7 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
                        size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
                        static_cast<uint32_t*>(j.status), zero_words, nullptr);
   if (clk) clk->end(s);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(KB, LEARNED)));
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(KB, LEARNED, vals)));
   hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk)
                                   : run_sweeps<U, uint32_t, LEARNED>(j, s, clk);
   if (e != hipSuccess) return e;
@@ -677,7 +690,57 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   return hipGetLastError();
 }
 
+// Ranks eight digit patterns (uniform bytes, two values, sixteen values, one
+// value, one digit per lane, pairs of lanes, 90 % one digit, runs of eight)
+// both with LDS atomics and with ballots and adds the number of keys whose
+// ranks differ to *bad.  Block b uses pattern b % 8.
+__global__ __launch_bounds__(512) void k_rank_check(uint32_t* bad) {
+  constexpr int IPT = 16;
+  __shared__ uint32_t ha[8][256], hb[8][256];
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (unsigned i = tid; i < 8 * 256; i += 512) {
+    (&ha[0][0])[i] = 0;
+    (&hb[0][0])[i] = 0;
+  }
+  __syncthreads();
+  const unsigned pattern = blockIdx.x & 7u;
+  uint32_t k[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    uint32_t x = (blockIdx.x * 512u + tid) * 0x9E3779B1u + uint32_t(j) * 0x85EBCA77u;
+    x ^= x >> 15;
+    x *= 0x2C1B3C6Du;
+    x ^= x >> 12;
+    uint32_t d;
+    switch (pattern) {
+      case 0: d = x; break;
+      case 1: d = (x & 1u) ? 7u : 200u; break;
+      case 2: d = (x & 15u) << 4; break;
+      case 3: d = 42u; break;
+      case 4: d = lane; break;
+      case 5: d = lane >> 1; break;
+      case 6: d = (x % 10u) ? 3u : (x >> 8); break;
+      default: d = ((uint32_t(j) * 64u + lane) >> 3) * 37u; break;
+    }
+    k[j] = d & 255u;
+  }
+  uint32_t ra[IPT / 2], rb[IPT / 2];
+  rank_in_wave_atomic<uint32_t, IPT, true>(k, ra, 0u, ha[w], 0u, 0u);
+  rank_in_wave<uint32_t, IPT, true>(k, rb, 0u, hb[w], 0u, 0u);
+  uint32_t diff = 0;
+#pragma unroll
+  for (int q = 0; q < IPT / 2; ++q) diff += (ra[q] & 0xFFFFu) != (rb[q] & 0xFFFFu);
+#pragma unroll
+  for (int q = 0; q < IPT / 2; ++q) diff += (ra[q] >> 16) != (rb[q] >> 16);
+  if (diff) atomicAdd(bad, diff);
+}
+
 }  // namespace
+
+hipError_t rank_order_check(uint32_t* bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_rank_check, dim3(64), dim3(512), 0, s, bad);
+  return hipGetLastError();
+}
 
 hipError_t radix_sort(const RadixJob& job, hipStream_t s, KernelClock* clk) {
   if (job.model) {
