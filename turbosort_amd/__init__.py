@@ -118,7 +118,7 @@ class Context:
 
     def kernel_times(self):
         """[(kernel name, ms, algorithmic bytes)] of the last call, from HIP events."""
-        cap = 64
+        cap = 256
         names = (ctypes.c_char_p * cap)()
         ms = (ctypes.c_float * cap)()
         by = (ctypes.c_double * cap)()

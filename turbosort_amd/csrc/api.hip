@@ -309,6 +309,7 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
       gj.counts = d_counts;
       gj.lists = d_lists;
       gj.num_cus = c->num_cus;
+      gj.atomic_rank = c->atomic_rank;
       set_variants(c, kb, false, gj);
       if (e == hipSuccess) e = ts::segmented_sort(gj, s, &c->clk);
       if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
@@ -555,6 +556,7 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
   j.counts = reinterpret_cast<uint32_t*>(c->ws);
   j.lists = reinterpret_cast<uint32_t*>(c->ws + count_bytes);
   j.num_cus = c->num_cus;
+  j.atomic_rank = c->atomic_rank;
   set_variants(c, kb, vals != nullptr, j);
   hipError_t e = ts::segmented_sort(j, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
@@ -693,6 +695,7 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   sj.counts = reinterpret_cast<uint32_t*>(w + off[12]);
   sj.lists = reinterpret_cast<uint32_t*>(w + off[13]);
   sj.num_cus = c->num_cus;
+  sj.atomic_rank = c->atomic_rank;
   set_variants(c, kb, false, sj);
   e = ts::segmented_sort(sj, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");

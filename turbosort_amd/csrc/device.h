@@ -124,13 +124,32 @@ __device__ __forceinline__ void rank_in_wave(const U (&k)[IPT], uint32_t (&rk)[(
 // LDS that the context checks once per device before using this ranking),
 // and a wave's instructions in program order, so ranks follow the key order
 // j * 64 + lane, as stability needs.  Lanes past the tile's end add nothing.
+// Adds to one counter from one instruction are applied one after another, so
+// a digit that at least 16 of the wave's keys share (presorted or low-entropy
+// data) is counted first by one add of its lane count, and its lanes take
+// their ranks from that base and their lane order among themselves.
 template <class U, int IPT, bool FULL>
 __device__ __forceinline__ void rank_in_wave_atomic(const U (&k)[IPT], uint32_t (&rk)[(IPT + 1) / 2], unsigned shift,
                                                     uint32_t* whist, uint32_t lb, uint32_t valid) {
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
+    const uint32_t d = digit_of(k[j], shift);
+    const bool in = FULL || lb + uint32_t(j) * 64 < valid;
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const uint64_t same = __ballot(in && d == d0);
     uint32_t r = 0;
-    if (FULL || lb + uint32_t(j) * 64 < valid) r = atomicAdd(&whist[digit_of(k[j], shift)], 1u);
+    if (__popcll(same) >= 16) {
+      const uint32_t leader = uint32_t(__builtin_ctzll(same));
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&whist[d0], uint32_t(__popcll(same)));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if ((same >> lane_id()) & 1u)
+        r = base + __builtin_amdgcn_mbcnt_hi(uint32_t(same >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(same), 0u));
+      else if (in)
+        r = atomicAdd(&whist[d], 1u);
+    } else if (in) {
+      r = atomicAdd(&whist[d], 1u);
+    }
     if (j & 1)
       rk[j >> 1] |= r << 16;
     else

@@ -151,6 +151,7 @@ struct SortMeta {
   uint32_t seg_slot[8][kSegments];
   uint32_t total_tiles[8];
   uint32_t tickets[8];
+  uint32_t seg_tickets[8][kSegments];  // per pass and segment: rounds claimed (XCD-affine claims)
   uint32_t n_live;
   uint32_t live[8];
   uint32_t error;  // non-zero: a look-back wait hit its bound (never expected)

@@ -59,7 +59,7 @@ struct KernelClock {
 #define TS_SWEEP_LEARNED_NARROW 512, 32
 #endif
 #ifndef TS_SWEEP_LEARNED_WIDE
-#define TS_SWEEP_LEARNED_WIDE 512, 14
+#define TS_SWEEP_LEARNED_WIDE 1024, 14
 #endif
 // Predecessor tiles one look-back step reads at once.
 #ifndef TS_LOOKBACK_WINDOW
@@ -170,6 +170,7 @@ struct SegJob {
   uint32_t* lists = nullptr;   // kSegClasses * n_segments words
   int num_cus = 256;
   uint8_t variant[kSegClasses] = {};  // kernel shape per size class (0: default)
+  bool atomic_rank = false;           // as RadixJob::atomic_rank
 };
 uint32_t seg_lds_max_len(int key_bytes, bool vals);
 // Kernel shapes a size class can run (variant 0 is the default), and a name

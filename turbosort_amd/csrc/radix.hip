@@ -35,6 +35,7 @@ template <class U> __device__ __forceinline__ U first_lane(U x) {
 __global__ void k_init(SortMeta* m, int P) {
   for (unsigned i = threadIdx.x; i < unsigned(P) * kSegments * 256; i += blockDim.x) (&m->pair[0][0][0])[i] = 0;
   if (threadIdx.x < 8) m->tickets[threadIdx.x] = 0;
+  if (threadIdx.x < 8 * kSegments) (&m->seg_tickets[0][0])[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     m->n_live = 0;
     m->error = 0;
@@ -284,11 +285,16 @@ template <class U> struct SweepArgs {
   const SortMeta* meta;
   const PassGeo* geo;  // meta->geo, read-only in the scatter
   uint32_t* tickets;
+  uint32_t* seg_tickets;  // [pass][segment]
   uint32_t* error;  // set when a look-back wait exceeds its bound
   uint32_t num_tiles;
   const LearnedModel* model;  // learned partition: digits of the key's bucket
   int atomic_rank;            // rank with LDS atomics (rank_in_wave_atomic), else ballots
 };
+
+#ifndef TS_SWEEP_XCD_AFFINE
+#define TS_SWEEP_XCD_AFFINE 1
+#endif
 
 // About two seconds of s_sleep(1) polling.
 constexpr uint32_t kSpinLimit = 1u << 26;
@@ -554,7 +560,7 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS>::threads)) vo
   const uint32_t n_live = meta->n_live;
   if (uint32_t(pass) >= n_live) return;
   const unsigned tid = threadIdx.x;
-  if (tid == 0) L.ticket = atomicAdd(&a.tickets[pass], 1u);
+  if (!TS_SWEEP_XCD_AFFINE && tid == 0) L.ticket = atomicAdd(&a.tickets[pass], 1u);
   // The pass's geometry, from wave-uniform addresses (scalar loads) issued
   // while the ticket is in flight.
   const PassGeo* __restrict__ g = a.geo + pass;
@@ -570,15 +576,47 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS>::threads)) vo
   for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
   if (tid < 256) L.tcnt[tid] = 0;
   if (LEARNED) load_knots(L.knots, a.model);
-  __syncthreads();
-  const uint32_t ticket = __builtin_amdgcn_readfirstlane(L.ticket);
-  S* next_row = status + size_t((pass + 1) & 1) * a.num_tiles * 256;
-  if (tid < 256) next_row[size_t(ticket) * 256 + tid] = 0;  // the next pass's look-back row
-  if (ticket >= total) return;
   uint32_t seg, round;
-  ticket_to_tile(ticket, nt, &seg, &round);  // uniform: stays in scalar registers
-  seg = __builtin_amdgcn_readfirstlane(seg);
-  round = __builtin_amdgcn_readfirstlane(round);
+  S* next_row = status + size_t((pass + 1) & 1) * a.num_tiles * 256;
+  if (TS_SWEEP_XCD_AFFINE) {
+    // Claim a round of the segment numbered like this workgroup's XCD, so
+    // that a segment's consecutive tiles, whose digit runs abut, write
+    // through one L2; once it is exhausted, of the next segments in turn.
+    // Within a segment rounds are claimed in order, so a tile's predecessor
+    // was claimed first by a running workgroup.
+    if (tid == 0) {
+      uint32_t x;
+      asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+      uint32_t got = kSegments, r = 0;
+      for (uint32_t i = 0; i < uint32_t(kSegments); ++i) {
+        const uint32_t sg = (x + i) & uint32_t(kSegments - 1);
+        if (nt[sg] == 0) continue;
+        r = atomicAdd(&a.seg_tickets[pass * kSegments + sg], 1u);
+        if (r < nt[sg]) {
+          got = sg;
+          break;
+        }
+      }
+      L.ticket = (got << 24) | r;
+    }
+    __syncthreads();
+    const uint32_t claim = __builtin_amdgcn_readfirstlane(L.ticket);
+    // Rows of the next pass's look-back words: a claimed tile clears its
+    // slot's, and workgroup b clears row b when b is no tile's slot.
+    if (tid < 256 && blockIdx.x >= total && blockIdx.x < a.num_tiles) next_row[size_t(blockIdx.x) * 256 + tid] = 0;
+    seg = claim >> 24;
+    if (seg >= uint32_t(kSegments)) return;
+    round = claim & 0xFFFFFFu;
+    if (tid < 256) next_row[(size_t(g->slot[seg]) + round) * 256 + tid] = 0;
+  } else {
+    __syncthreads();
+    const uint32_t ticket = __builtin_amdgcn_readfirstlane(L.ticket);
+    if (tid < 256) next_row[size_t(ticket) * 256 + tid] = 0;  // the next pass's look-back row
+    if (ticket >= total) return;
+    ticket_to_tile(ticket, nt, &seg, &round);  // uniform: stays in scalar registers
+    seg = __builtin_amdgcn_readfirstlane(seg);
+    round = __builtin_amdgcn_readfirstlane(round);
+  }
 
   const size_t seg_lo = g->start[seg];
   const size_t seg_hi = seg_lo + g->count[seg];
@@ -627,6 +665,7 @@ template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& 
   a.meta = j.meta;
   a.geo = j.meta->geo;
   a.tickets = j.meta->tickets;
+  a.seg_tickets = &j.meta->seg_tickets[0][0];
   a.error = &j.meta->error;
   a.num_tiles = tiles;
   a.model = j.model;

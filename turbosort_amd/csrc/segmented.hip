@@ -41,6 +41,7 @@ struct SegArgs {
   uint32_t* lists;   // [kSegClasses][n_segments] segment ids
   uint32_t caps[kSegClasses];
   int mode;
+  int atomic_rank;  // byte-digit ranks by LDS atomic adds (rank_in_wave_atomic), else ballots
 };
 
 __device__ __forceinline__ unsigned long long seg_end(const SegArgs& a, size_t s) {
@@ -493,7 +494,10 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
       if (((varying >> shift) & 0xFFu) == 0) continue;  // every key agrees on this digit
       for (unsigned x = lane; x < 256; x += 64) L.whist[w][x] = 0;  // this wave's row only
       uint32_t rk[(IPT + 1) / 2];
-      rank_in_wave<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
+      if (a.atomic_rank)
+        rank_in_wave_atomic<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
+      else
+        rank_in_wave<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
 #pragma unroll
       for (int j = 0; j < IPT; ++j) opaque(k[j]);  // recompute digits below rather than keep them
       __syncthreads();
@@ -821,6 +825,7 @@ template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStrea
   a.counts = j.counts;
   a.lists = j.lists;
   a.mode = j.mode;
+  a.atomic_rank = j.atomic_rank ? 1 : 0;
   for (int c = 0; c < kSegClasses; ++c) a.caps[c] = c < kSegDevice ? class_cap<U, VALS>(c) : 0u;
   hipLaunchKernelGGL(k_seg_clear, dim3(1), dim3(64), 0, s, j.counts);
   const size_t per_block = size_t(kClassifyThreads) * kClassifyItems;
