@@ -182,20 +182,23 @@ struct LearnedModel {
   uint32_t bucket_shift;       // 32 - log2(number of buckets)
   uint32_t linear;             // 1: coordinate d >> shift; 0: log2_fixed(d)
   uint32_t shift;
+  uint32_t cmul, cshift;    // u = x * cmul >> cshift stretches [0, xhi] onto [0, 2^29)
   unsigned long long kmin;  // smallest sample key
-  unsigned long long xhi;   // coordinate of the largest sample key
-  unsigned long long mul;   // floor(2^61 / (xhi + 1))
+  unsigned long long xhi;   // coordinate of the largest sample key (below 2^32)
 };
 
-__host__ __device__ __forceinline__ unsigned long long log2_fixed(unsigned long long k) {
-  if (k == 0) return 0;
 #if defined(__HIP_DEVICE_COMPILE__)
-  const int e = 63 - __clzll((long long)k);
+#define TS_CLZ64(x) __clzll((long long)(x))
 #else
-  const int e = 63 - __builtin_clzll(k);
+#define TS_CLZ64(x) __builtin_clzll(x)
 #endif
-  const unsigned long long frac = e >= 23 ? (k >> (e - 23)) : (k << (23 - e));
-  return (unsigned long long)e << 23 | (frac & 0x7FFFFFull);
+
+// e * 2^23 + (the 23 bits below k's leading one, zero-filled), e the
+// leading one's position; below 2^29.
+__host__ __device__ __forceinline__ uint32_t log2_fixed(unsigned long long k) {
+  if (k == 0) return 0;
+  const int lz = TS_CLZ64(k);
+  return uint32_t(63 - lz) << 23 | uint32_t((k << lz) >> 40) & 0x7FFFFFu;
 }
 
 __host__ __device__ __forceinline__ unsigned long long learned_x(unsigned long long key, const LearnedModel& m) {
@@ -203,11 +206,23 @@ __host__ __device__ __forceinline__ unsigned long long learned_x(unsigned long l
   return m.linear ? d >> m.shift : log2_fixed(d);
 }
 
+// cmul and cshift for a coordinate range [0, xhi], xhi < 2^32:
+// cmul = floor(2^(61 - t) / (xhi + 1)) and cshift = 32 - t, with the
+// smallest t >= 0 that keeps cmul below 2^32 for every xhi of its bit length,
+// so that x * cmul >> cshift < 2^29 for every x <= xhi (a 32 x 32-bit
+// product).
+__host__ __device__ __forceinline__ void learned_scale(unsigned long long xhi, uint32_t* cmul, uint32_t* cshift) {
+  const int len = 64 - TS_CLZ64(xhi + 1);  // 2^(len-1) <= xhi + 1 < 2^len
+  const int t = len >= 31 ? 0 : 31 - len;
+  *cmul = uint32_t((1ull << (61 - t)) / (xhi + 1));
+  *cshift = uint32_t(32 - t);
+}
+
 // The model coordinate of a radix key: u in [0, kKnots << kKnotFracBits).
 __host__ __device__ __forceinline__ uint32_t learned_coord(unsigned long long key, const LearnedModel& m) {
   unsigned long long x = learned_x(key, m);
   x = x > m.xhi ? m.xhi : x;
-  return uint32_t((x * m.mul) >> 32);
+  return uint32_t((unsigned long long)uint32_t(x) * m.cmul >> m.cshift);
 }
 
 __host__ __device__ __forceinline__ uint32_t learned_cdf(uint32_t u, const uint32_t* knots) {

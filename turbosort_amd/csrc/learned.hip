@@ -57,14 +57,15 @@ __global__ void k_learn_points(const unsigned long long* __restrict__ sorted, si
     const int bits = span ? 64 - __clzll((long long)span) : 0;
     m.shift = linear && bits > 32 ? uint32_t(bits - 32) : 0u;
     m.xhi = learned_x(sorted[S - 1], m);
-    m.mul = (1ull << 61) / (m.xhi + 1);
+    learned_scale(m.xhi, &m.cmul, &m.cshift);
     m.bucket_shift = bucket_shift;
     if (blockIdx.x == 0) {
       model->kmin = m.kmin;
       model->linear = m.linear;
       model->shift = m.shift;
       model->xhi = m.xhi;
-      model->mul = m.mul;
+      model->cmul = m.cmul;
+      model->cshift = m.cshift;
       model->bucket_shift = bucket_shift;
     }
   }
