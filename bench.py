@@ -38,8 +38,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="u32", choices=sorted(WORKLOADS))
     p.add_argument("--log2n", type=int, default=28)
-    p.add_argument("--also", default="u64,u64z,segmented,by_key,learned",
-                   help="extra workloads reported beside the headline (u32, u64, u64z, segmented, by_key, learned)")
+    p.add_argument("--also", default="u64,u64z,host,segmented,by_key,learned",
+                   help="extra workloads reported beside the headline (u32, u64, u64z, host, segmented, by_key, learned)")
     p.add_argument("--cpu-seconds", type=float, default=8.0, help="wall budget of the CPU baseline")
     p.add_argument("--no-cpu-baseline", action="store_true")
     return p.parse_args()
@@ -304,6 +304,40 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5):
                 apply_order_ms=dt_apply * 1e3, apply_order_GBps=20 * n / dt_apply / 1e9, ok=ok)
 
 
+def run_host(steps: int, ctx, log2n: int = 28, seed: int = 2):
+    """sort_host on 2^28 u32 keys in pinned host memory: H2D + sort + D2H, the
+    end-to-end rate for a caller whose slice lives in host memory."""
+    import torch
+
+    import turbosort_amd as ts
+
+    n = 1 << log2n
+    dev = torch.device("cuda", 0)
+    d = torch.empty(n, dtype=torch.uint32, device=dev)
+    ts.fill_splitmix64(d, seed, ctx=ctx)
+    src = torch.empty(n, dtype=torch.uint32, pin_memory=True)
+    src.copy_(d)
+    before = ts.checksum(d, ctx=ctx)
+    h = torch.empty_like(src).pin_memory()
+    h.copy_(src)
+    ts.sort_host(h, ctx=ctx)  # warm: the context's device buffer grows once
+    times = []
+    for _ in range(steps):
+        h.copy_(src)
+        t0 = time.perf_counter()
+        ts.sort_host(h, ctx=ctx)
+        times.append(time.perf_counter() - t0)
+    d.copy_(h)
+    after = ts.checksum(d, ctx=ctx)
+    ok = after[0] == before[0] and after[1] == before[1] and after[2] == 0
+    t = sorted(times)[len(times) // 2]
+    del d, src, h
+    torch.cuda.empty_cache()
+    return dict(desc="sort_host: 2^28 u32 keys (SplitMix64 state 2) in pinned host memory; copy to the device, "
+                     "sort, copy back (wall time, median)", n=n, ms_per_step=t * 1e3, keys_per_s=n / t,
+                link_GBps_both_ways=2 * 4 * n / t / 1e9, ok=ok)
+
+
 GAMMA = 0x9E3779B97F4A7C15
 
 
@@ -485,6 +519,8 @@ def main():
             r = run_by_key(max(3, a.steps // 3), ctx)
         elif name == "learned":
             r = run_learned(max(2, a.steps // 5), ctx)
+        elif name == "host":
+            r = run_host(max(3, a.steps // 3), ctx)
         else:
             r = run_workload(name, a.log2n, max(3, a.steps // 3), 1, ctx)
             r["roofline_frac"] = r["total_gbs"] / HBM_PEAK_GBS

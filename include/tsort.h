@@ -66,6 +66,13 @@ ts_status ts_sort(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, void* str
 ts_status ts_sort_with_buffer(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, void* d_buf,
                               size_t buf_len, void* stream);
 
+/* turbosort::sort (lib.rs:99-101) on a slice in host memory: copies it to the device
+ * (in 64 MiB pieces; a pinned slice streams at the link rate, a pageable one
+ * through the runtime's staging), sorts it there and copies it back, and
+ * returns once the host slice is sorted.  The device copy is a grow-only
+ * buffer of the context, so repeated calls allocate nothing once warm. */
+ts_status ts_sort_host(ts_ctx* ctx, ts_dtype dtype, void* host_data, size_t n, void* stream);
+
 /* turbosort::sort_by_key_with_buffer (lib.rs:196-202, by_key.rs:175-235) for
  * POD records: record i starts at d_records + i*stride and its key, of type
  * key_type, at byte key_offset.  Writes d_order_out[i] = original position of

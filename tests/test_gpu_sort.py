@@ -187,3 +187,21 @@ def test_both_ranking_methods(gpu):
         want = np.argsort(dup, kind="stable").astype(np.uint32)
         assert np.array_equal(v.cpu().numpy().astype(np.uint32), want), method
         assert np.array_equal(to_host(k, np.uint32), dup[want]), method
+
+
+@pytest.mark.parametrize("pinned", [False, True])
+def test_sort_host(gpu, pinned):
+    """sort_host: a host slice is copied in, sorted on the device and copied
+    back (lib.rs:99-101 on host memory), pageable or pinned, reusing the
+    context's device buffer across sizes."""
+    ctx = ts.Context(gpu)
+    for name, n in (("u32", 0), ("u32", 1), ("f32", 2), ("u64", 4097), ("i16", 70_000), ("f64", 300_000),
+                    ("u32", 5_000_000), ("u8", 1000)):
+        x = random_of(name, n, seed=n + 3)
+        t = torch.from_numpy(x.copy())
+        if pinned:
+            t = t.pin_memory()
+        ts.sort_host(t, ctx=ctx)
+        assert np.array_equal(bits(t.numpy()), bits(cf.sort(x))), (name, n, pinned)
+    with pytest.raises(ValueError):
+        ts.sort_host(torch.zeros(4, dtype=torch.uint32, device=gpu), ctx=ctx)

@@ -36,7 +36,7 @@ import torch
 from . import _native
 
 __all__ = [
-    "TurbosortPanic", "Context", "KeyField", "sort", "sort_with_buffer", "sort_by_key",
+    "TurbosortPanic", "Context", "KeyField", "sort", "sort_with_buffer", "sort_host", "sort_by_key",
     "sort_by_key_with_buffer", "SortByKeyBuffer", "fill_splitmix64", "checksum", "DTYPE_CODES",
 ]
 
@@ -184,6 +184,21 @@ def sort_with_buffer(t: torch.Tensor, buffer: torch.Tensor, *, ctx: Optional[Con
     ctx = ctx or _default_context(t.device)
     ctx.check(ctx._lib.ts_sort_with_buffer(ctx.handle, code, ctypes.c_void_p(t.data_ptr()), t.numel(),
                                            ctypes.c_void_p(buffer.data_ptr()), buffer.numel(), _stream(t.device)))
+
+
+def sort_host(t: torch.Tensor, *, ctx: Optional[Context] = None) -> None:
+    """Sort a 1-D CPU tensor of a SortableKey type in place (lib.rs:99-101)
+    on the GPU: copied to the device, sorted there and copied back; returns
+    once ``t`` is sorted.  A pinned tensor (``pin_memory()``) streams at the
+    link rate."""
+    if t.device.type != "cpu":
+        raise ValueError("sort_host takes a CPU tensor; use sort for device tensors")
+    if t.dim() != 1 or not t.is_contiguous():
+        raise ValueError("slice must be a contiguous 1-D tensor")
+    code = _dtype_code(t.dtype)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    ctx = ctx or _default_context(dev)
+    ctx.check(ctx._lib.ts_sort_host(ctx.handle, code, ctypes.c_void_p(t.data_ptr()), t.numel(), _stream(ctx.device)))
 
 
 @dataclass(frozen=True)

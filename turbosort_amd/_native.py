@@ -20,7 +20,7 @@ EXPORTS = [
     "ts_sbk_last_error", "ts_segmented_sort_pairs", "ts_segmented_lds_max_len",
     "ts_learned_cfg_default", "ts_learned_sort", "ts_pwl_fit",
     "ts_segmented_variants", "ts_segmented_variant_name", "ts_segmented_set_variant",
-    "ts_ctx_set_ranking", "ts_ctx_ranking",
+    "ts_ctx_set_ranking", "ts_ctx_ranking", "ts_sort_host",
 ]
 
 
@@ -102,6 +102,8 @@ def load() -> ctypes.CDLL:
     L.ts_checksum.restype = i32
     L.ts_ctx_set_profiling.argtypes = [vp, i32]
     L.ts_ctx_set_profiling.restype = i32
+    L.ts_sort_host.argtypes = [vp, i32, vp, ctypes.c_size_t, vp]
+    L.ts_sort_host.restype = i32
     L.ts_ctx_set_ranking.argtypes = [vp, i32]
     L.ts_ctx_set_ranking.restype = i32
     L.ts_ctx_ranking.argtypes = [vp]

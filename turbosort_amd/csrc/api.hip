@@ -31,6 +31,9 @@ struct ts_ctx {
   // lane order (rank_order_check), and whether the sorts use that ranking.
   bool atomic_rank_ok = false;
   bool atomic_rank = false;
+  // ts_sort_host: device copy of the caller's host slice (grow-only).
+  char* hdev = nullptr;
+  size_t hdev_bytes = 0;
 };
 
 namespace {
@@ -404,6 +407,7 @@ void ts_ctx_destroy(ts_ctx* c) {
   if (!c) return;
   DeviceScope ds(c->device);
   if (c->ws) hipFree(c->ws);
+  if (c->hdev) hipFree(c->hdev);
   if (c->clk.made)
     for (int i = 0; i < 2 * ts::KernelClock::kMax; ++i) hipEventDestroy(c->clk.ev[i]);
   delete c;
@@ -439,6 +443,46 @@ ts_status ts_sort_with_buffer(ts_ctx* c, ts_dtype dt, void* data, size_t n, void
   if (n > 1 && (!data || !buf)) return fail(c, TS_ERR_INVALID_ARG, "null data or buffer");
   DeviceScope ds(c->device);
   return sort_impl(c, dt, data, n, buf, static_cast<hipStream_t>(stream));
+}
+
+ts_status ts_sort_host(ts_ctx* c, ts_dtype dt, void* host_data, size_t n, void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(dt)) return fail(c, TS_ERR_INVALID_ARG, "unknown dtype");
+  if (n <= 1) return TS_OK;
+  if (!host_data) return fail(c, TS_ERR_INVALID_ARG, "null data");
+  DeviceScope ds(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bytes = n * size_t(ts::dtype_bytes(dt));
+  if (bytes > c->hdev_bytes) {
+    if (c->hdev) {
+      hipError_t e = hipFree(c->hdev);
+      c->hdev = nullptr;
+      c->hdev_bytes = 0;
+      if (e != hipSuccess) return hip_fail(c, e, "hipFree");
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&c->hdev), bytes);
+    if (e != hipSuccess) {
+      c->hdev = nullptr;
+      return hip_fail(c, e, "hipMalloc(host-sort buffer)");
+    }
+    c->hdev_bytes = bytes;
+  }
+  // Copies in chunks so that a pinned slice streams at full link rate and a
+  // pageable one goes through the runtime's staging piece by piece.
+  constexpr size_t kChunk = size_t(64) << 20;
+  char* h = static_cast<char*>(host_data);
+  for (size_t o = 0; o < bytes; o += kChunk) {
+    hipError_t e = hipMemcpyAsync(c->hdev + o, h + o, std::min(kChunk, bytes - o), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpyAsync(host to device)");
+  }
+  ts_status st = sort_impl(c, dt, c->hdev, n, nullptr, s);
+  if (st != TS_OK) return st;
+  for (size_t o = 0; o < bytes; o += kChunk) {
+    hipError_t e = hipMemcpyAsync(h + o, c->hdev + o, std::min(kChunk, bytes - o), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpyAsync(device to host)");
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "hipStreamSynchronize");
 }
 
 ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t stride, size_t key_offset,
