@@ -128,3 +128,17 @@ def test_generic_element_sizes_for_apply_order(gpu):
         t = torch.from_numpy(other.copy()).to(gpu)
         buf.apply_order(t)
         assert np.array_equal(t.cpu().numpy(), other[order]), width
+
+
+def test_presorted_and_descending_keys_keep_the_stable_order(gpu):
+    """Ascending keys give the identity order; non-increasing keys with ties
+    are not reversed by key-value sorts (ties keep their input order)."""
+    n = 300_000
+    asc = np.sort(random_of("u32", n, seed=5) % 5000).astype(np.uint32)
+    for keys in (asc, asc[::-1].copy()):
+        t = to_dev(keys, gpu)
+        buf = ts.SortByKeyBuffer()
+        ts.sort_by_key_with_buffer(t, buf)
+        want = np.argsort(keys, kind="stable").astype(np.uint32)
+        assert np.array_equal(to_host(buf.order(), np.uint32), want)
+        assert np.array_equal(to_host(t, np.uint32), keys[want])

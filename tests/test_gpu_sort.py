@@ -205,3 +205,24 @@ def test_sort_host(gpu, pinned):
         assert np.array_equal(bits(t.numpy()), bits(cf.sort(x))), (name, n, pinned)
     with pytest.raises(ValueError):
         ts.sort_host(torch.zeros(4, dtype=torch.uint32, device=gpu), ctx=ctx)
+
+
+@pytest.mark.parametrize("n", [4097, 100_000, 1_000_003])
+def test_presorted_shortcuts(gpu, n):
+    """Non-decreasing input skips every pass and non-increasing input is
+    reversed (presorted.rs:15-65); any single out-of-place pair, including at
+    the ends and at workgroup-chunk boundaries, must fall back to the sort."""
+    g = rng(n)
+    for name in ("u32", "u64", "f32", "i16", "f64"):
+        x = cf.sort(random_of(name, n, seed=n + 1))
+        check(x, gpu, f"{name} ascending")
+        check(x[::-1].copy(), gpu, f"{name} descending")
+        ties = np.repeat(x[: n // 3], 3)[:n]
+        check(ties[::-1].copy(), gpu, f"{name} descending with ties")
+        for pos in {1, n // 2, n - 1, int(g.integers(1, n - 1)), 65_536 if n > 65_536 else n // 3}:
+            y = x.copy()
+            y[pos - 1], y[pos] = y[pos], y[pos - 1]
+            check(y, gpu, f"{name} ascending, pair {pos} swapped")
+            z = x[::-1].copy()
+            z[pos - 1], z[pos] = z[pos], z[pos - 1]
+            check(z, gpu, f"{name} descending, pair {pos} swapped")

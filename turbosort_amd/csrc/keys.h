@@ -155,7 +155,11 @@ struct SortMeta {
   uint32_t n_live;
   uint32_t live[8];
   uint32_t error;  // non-zero: a look-back wait hit its bound (never expected)
-  uint32_t pad[14];
+  // Presorted input (presorted.rs:15-65): set by the histogram read when some
+  // adjacent pair of radix keys decreases / increases; reverse: the scan
+  // found the keys non-increasing and a reverse finishes the sort.
+  uint32_t breaks_up, breaks_down, reverse;
+  uint32_t pad[11];
   unsigned long long check[4];  // checksum scratch: sum, xor, unsorted pairs
 };
 

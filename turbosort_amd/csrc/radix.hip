@@ -39,6 +39,9 @@ __global__ void k_init(SortMeta* m, int P) {
   if (threadIdx.x == 0) {
     m->n_live = 0;
     m->error = 0;
+    m->breaks_up = 0;
+    m->breaks_down = 0;
+    m->reverse = 0;
   }
 }
 
@@ -98,15 +101,69 @@ __device__ __forceinline__ uint32_t bucket_of(U k, const uint32_t* knots, const 
   return learned_cdf(learned_coord((unsigned long long)k, m), knots) >> shift;
 }
 
+// Presorted scan (presorted.rs:15-65) on the device: flags whether some
+// adjacent pair of radix keys decreases (breaks_up) and whether some pair
+// increases (breaks_down).  Workgroup b checks the pairs (i-1, i) of its
+// contiguous chunk, each lane 16 consecutive keys per step, and stops as
+// soon as it has seen both kinds of break itself or, looking every 16 steps,
+// finds both flags set: random input ends within a step, and only ordered
+// (or almost ordered) input is read in full.
+constexpr int kOrderKeys = 16;
+#ifndef TS_ORDER_BLOCKS_PER_CU
+#define TS_ORDER_BLOCKS_PER_CU 1
+#endif
+template <class U, bool REC>
+__global__ __launch_bounds__(256) void k_order_check(const U* __restrict__ keys, const unsigned char* __restrict__ rec,
+                                                     size_t stride, size_t key_off, size_t n, int mode,
+                                                     SortMeta* m) {
+  const size_t per = (n + gridDim.x - 1) / gridDim.x;
+  const size_t b0 = max(size_t(1), size_t(blockIdx.x) * per), b1 = min(n, size_t(blockIdx.x + 1) * per);
+  auto key = [&](size_t i) {
+    const U x = REC ? *reinterpret_cast<const U*>(rec + i * stride + key_off) : keys[i];
+    return to_key(x, mode);
+  };
+  bool up = false, down = false;
+  uint32_t step = 0;
+  for (size_t base = b0; base < b1; base += size_t(256) * kOrderKeys, ++step) {
+    const size_t i0 = base + size_t(threadIdx.x) * kOrderKeys;
+    U k[kOrderKeys + 1];
+#pragma unroll
+    for (int r = 0; r <= kOrderKeys; ++r) k[r] = key(min(i0 - 1 + size_t(r), n - 1));
+#pragma unroll
+    for (int r = 1; r <= kOrderKeys; ++r) {
+      if (i0 - 1 + size_t(r) < b1) {
+        up |= k[r - 1] > k[r];
+        down |= k[r - 1] < k[r];
+      }
+    }
+    up = __syncthreads_or(up);
+    down = __syncthreads_or(down);
+    if (up && down) break;
+    if ((step & 15) == 15) {
+      const bool both = __hip_atomic_load(&m->breaks_up, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+                        __hip_atomic_load(&m->breaks_down, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__syncthreads_or(both)) return;  // the same answer in every lane
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (up) atomicOr(&m->breaks_up, 1u);
+    if (down) atomicOr(&m->breaks_down, 1u);
+  }
+}
+
 template <class U, bool REC, bool LEARNED = false>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ keys, const unsigned char* __restrict__ rec,
                                                        size_t stride, size_t key_off, U* __restrict__ keys_out,
                                                        size_t n, int mode, size_t seg0, uint32_t blocks_per_seg,
                                                        size_t per_block, SortMeta* m, uint32_t* zero_ptr,
-                                                       size_t zero_words, const LearnedModel* model) {
+                                                       size_t zero_words, const LearnedModel* model,
+                                                       int check_order) {
   using D = typename DigitSource<U, LEARNED>::T;
   constexpr int P = DigitSource<U, LEARNED>::P;
   constexpr int UN = HistUnroll<U>::value;
+  // Keys already in order (k_order_check; check_order as k_scan's presorted):
+  // no pass runs, so no counts.
+  if (!LEARNED && check_order && (!m->breaks_up || (check_order == 2 && !m->breaks_down))) return;
   __shared__ uint32_t h[P * kSegments * 256];
   __shared__ uint32_t knots[LEARNED ? kKnots + 1 : 1];
   const unsigned tid = threadIdx.x, lane = tid & 63;
@@ -197,7 +254,12 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ key
 // segments collapse into one segment holding all keys in input order, which
 // is the plain single-chain pass, so the segments stay correct whichever
 // digits are live.
-__global__ __launch_bounds__(256) void k_scan(SortMeta* m, size_t n, int P, size_t seg0, uint32_t tile) {
+// presorted: 0 ignores the order check; 1 skips every pass when the keys are
+// already non-decreasing (finish_presorted, presorted.rs:15-27); 2 also
+// reverses non-increasing keys in place instead of sorting them (keys only:
+// equal keys are equal bits, presorted.rs:10-14).
+__global__ __launch_bounds__(256) void k_scan(SortMeta* m, size_t n, int P, size_t seg0, uint32_t tile,
+                                              int presorted) {
   __shared__ unsigned long long s_wave[4];
   __shared__ unsigned long long s_prev[256], s_prev_base[256];
   __shared__ uint32_t s_tiles[kSegments];
@@ -246,6 +308,12 @@ __global__ __launch_bounds__(256) void k_scan(SortMeta* m, size_t n, int P, size
     s_prev_base[tid] = e;
   }
   __syncthreads();
+  if (presorted >= 1 && m->breaks_up == 0) {
+    n_live = 0;
+  } else if (presorted == 2 && m->breaks_down == 0) {
+    n_live = 0;
+    if (tid == 0) m->reverse = 1;
+  }
   if (tid == 0) m->n_live = n_live;
   // Geometry of each executed pass, by execution order.
   for (uint32_t i = 0; i < n_live; ++i) {
@@ -635,15 +703,45 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS>::threads)) vo
                                            slot, round == 0, digit_base);
 }
 
-// An odd number of live passes leaves the keys in the scratch half.
+// Non-increasing keys found by the scan: reversing them sorts them.  Each
+// lane swaps 8 keys at each end per step (the grid's lanes striped over the
+// steps, so that every load and store is coalesced), loads before stores.
+template <class U> __device__ __forceinline__ void reverse_keys(U* __restrict__ a, size_t n) {
+  constexpr int R = 8;
+  const size_t half = n / 2;
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i0 = g; i0 < half; i0 += gs * R) {
+    U lo[R], hi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const size_t i = min(i0 + size_t(r) * gs, half - 1);
+      lo[r] = a[i];
+      hi[r] = a[n - 1 - i];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const size_t i = i0 + size_t(r) * gs;
+      if (i < half) {
+        a[i] = hi[r];
+        a[n - 1 - i] = lo[r];
+      }
+    }
+  }
+}
+
+// The last step of a keys-only sort: an odd number of live passes leaves the
+// keys in the scratch half, and keys the scan found non-increasing are
+// reversed.
 template <class U>
 __global__ void k_copy_back(const SortMeta* m, const U* __restrict__ src, U* __restrict__ dst, size_t n) {
+  if (m->reverse) return reverse_keys(dst, n);
   if ((m->n_live & 1u) == 0) return;
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
   for (size_t i = g; i < n; i += gs) dst[i] = src[i];
 }
 
-// With no live digit every key is equal: the stable order is the identity.
+// With no live digit every key is equal, or the keys were already in order:
+// the stable order is the identity.
 __global__ void k_iota_if_dead(const SortMeta* m, uint32_t* out, size_t n) {
   if (m->n_live != 0) return;
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
@@ -701,21 +799,36 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   const uint32_t tiles = sweep_tiles(j.n, KB, LEARNED, vals);
   const size_t zero_words = size_t(tiles) * 256 * (wide_status(j.n) ? 2 : 1);
   const dim3 hgrid(blocks_per_seg * kSegments);
+  // The order check does not apply to bucket digits, and only keys without
+  // values may be reversed (a stable order of non-increasing keys with ties
+  // is no reversal).
+  const bool vals_or_rec = j.records != nullptr || vals;
+  const int check_order = LEARNED ? 0 : vals_or_rec ? 1 : 2;
+  if constexpr (!LEARNED) {
+    const unsigned og = unsigned(std::max(1, j.num_cus * TS_ORDER_BLOCKS_PER_CU));
+    if (j.records)
+      hipLaunchKernelGGL((k_order_check<U, true>), dim3(og), dim3(256), 0, s, nullptr, j.records, j.stride,
+                         j.key_offset, j.n, j.mode, j.meta);
+    else
+      hipLaunchKernelGGL((k_order_check<U, false>), dim3(og), dim3(256), 0, s, static_cast<const U*>(j.keys_a),
+                         nullptr, size_t(0), size_t(0), j.n, j.mode, j.meta);
+  }
   if (clk) clk->begin(s, "k_hist", double(j.n) * (j.records ? double(j.stride) + KB : KB));
   if constexpr (LEARNED)
     hipLaunchKernelGGL((k_hist<U, false, true>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
                        nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                       static_cast<uint32_t*>(j.status), zero_words, j.model);
+                       static_cast<uint32_t*>(j.status), zero_words, j.model, check_order);
   else if (j.records)
     hipLaunchKernelGGL((k_hist<U, true>), hgrid, dim3(kHistThreads), 0, s, nullptr, j.records, j.stride,
                        j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                       static_cast<uint32_t*>(j.status), zero_words, nullptr);
+                       static_cast<uint32_t*>(j.status), zero_words, nullptr, check_order);
   else
     hipLaunchKernelGGL((k_hist<U, false>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a), nullptr,
                        size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                       static_cast<uint32_t*>(j.status), zero_words, nullptr);
+                       static_cast<uint32_t*>(j.status), zero_words, nullptr, check_order);
   if (clk) clk->end(s);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(KB, LEARNED, vals)));
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(KB, LEARNED, vals)),
+                     check_order);
   hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk)
                                   : run_sweeps<U, uint32_t, LEARNED>(j, s, clk);
   if (e != hipSuccess) return e;
