@@ -240,3 +240,48 @@ def test_autotuner_picks_a_variant_per_class(gpu):
     torch.cuda.synchronize()
     want_k, want_v = cf.segmented_sort(keys, off, vals)
     assert np.array_equal(to_host(tk, np.uint32), want_k) and np.array_equal(to_host(tv, np.uint32), want_v)
+
+
+@pytest.mark.parametrize("dtype", ["u32", "u64", "f64", "i32"])
+def test_msd_shapes_on_structured_segments(gpu, dtype):
+    """The keys-only MSD shape on segments that take each of its paths: short
+    runs (random keys), keys differing only in their low bits, every key equal,
+    few distinct values and a tight cluster plus one outlier (runs beyond 64
+    keys: the LSD fallback)."""
+    from turbosort_amd import autotune
+
+    tdt = {"u32": torch.uint32, "u64": torch.uint64, "f64": torch.float64, "i32": torch.int32}[dtype]
+    npdt = cf.NP_DTYPES[dtype]
+    udt = cf.UNSIGNED_OF[np.dtype(npdt).itemsize]
+    ctx = ts.Context(gpu)
+    g = rng(123)
+    for c in (6, 7, 8):
+        names = autotune.variants(tdt, False, c)
+        msd = [v for v, nm in enumerate(names) if "MSD" in nm]
+        assert len(msd) == 1, names
+        cap, lo = autotune.CLASS_CAPS[c], autotune.CLASS_CAPS[c - 1] + 1
+        parts = []
+        for kind in range(5):
+            m = int(g.integers(lo, cap + 1))
+            base = random_of(dtype, m, seed=c * 100 + kind)
+            if kind == 1:
+                base = np.sort(base)[:1].repeat(m)
+                base = (base.view(udt) ^ g.integers(0, 1024, m).astype(udt)).view(npdt)
+            elif kind == 2:
+                base = base[:1].repeat(m)
+            elif kind == 3:
+                base = base[g.integers(0, 5, m)]
+            elif kind == 4:
+                u = base.view(udt).copy()
+                u[:] = u[0] ^ g.integers(0, 1 << 16, m).astype(udt)
+                u[0] = ~u[1]
+                base = u.view(npdt)
+            parts.append(base.astype(npdt))
+        keys = np.concatenate(parts)
+        off = offsets_of([len(p) for p in parts])
+        want, _ = cf.segmented_sort(keys, off, None)
+        autotune.set_variant(ctx, tdt, False, c, msd[0])
+        tk = to_dev(keys, gpu)
+        ts.segmented_sort(tk, to_dev(off, gpu), None, max_segment_len=cap, ctx=ctx)
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(to_host(tk, keys.dtype)), bits(want)), (dtype, c)

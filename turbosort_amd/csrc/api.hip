@@ -386,6 +386,10 @@ ts_status ts_ctx_create(ts_ctx** out, int device) {
   c->device = device;
   c->atomic_rank_ok = ok;
   c->atomic_rank = ok;
+  // Keys-only segments of the largest class (learned buckets among them):
+  // the MSD shape by default (measured on MI355X: 27.4 vs 28.1 ms for the
+  // learned sort of 10^9 u64 keys).
+  for (int w = 2; w < 4; ++w) c->seg_variant[w][0][8] = 2;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     c->num_cus = cus;

@@ -547,6 +547,208 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
 }
 
 // ---------------------------------------------------------------------------
+// Keys-only segments by one most-significant-digit pass.  Without values no
+// order among equal keys is observable (equal keys are equal bits), so the
+// passes need not be stable: the keys (in registers) are counted by the top
+// kMsdBits bits of their offset from the segment's smallest key, the counts
+// scanned, and every key scattered into LDS at its digit's next free slot
+// (an LDS atomic add); each lane then insertion-sorts a few of the 4096 short
+// runs.  A segment with a run longer than kMsdRunMax keys (a cluster of
+// distinct keys: insertion sort would be quadratic in it) is finished by the
+// byte-digit LSD passes of k_seg_radix8 instead, over the bytes in which its
+// keys differ.
+constexpr int kMsdBits = 12, kMsdBins = 1 << kMsdBits, kMsdRunMax = 64;
+
+template <class U, int T, int IPT> struct SegMsdLds {
+  static constexpr int CAP = T * IPT;
+  alignas(16) U keys[CAP];
+  union {
+    uint32_t cnt[kMsdBins];     // run counts, then starts, then ends
+    uint32_t whist[T / 64][256];  // the LSD passes' per-wave digit counts
+  };
+  uint32_t wsum[T / 64];
+  uint32_t flags[3];  // bits in which the keys differ (low, high word), longest run
+  U kmin, kmax;
+};
+
+// Exclusive scan of one value per lane over all T lanes of the workgroup.
+template <int T> __device__ __forceinline__ uint32_t block_exclusive_add(uint32_t v, uint32_t* s_wave) {
+  const unsigned tid = threadIdx.x, w = tid >> 6;
+  const uint32_t inc = wave_inclusive_add(v);
+  if ((tid & 63) == 63) s_wave[w] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+#pragma unroll
+  for (int i = 0; i < T / 64; ++i)
+    if (uint32_t(i) < w) off += s_wave[i];
+  __syncthreads();
+  return off + inc - v;
+}
+
+template <class U, int T, int IPT>
+__global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
+  static_assert(T >= 256 && T % 64 == 0 && kMsdBins % T == 0, "lanes must divide the bins");
+  constexpr int NW = T / 64, PER = kMsdBins / T;
+  constexpr int P = int(sizeof(U));
+  constexpr int CAP = T * IPT;
+  __shared__ SegMsdLds<U, T, IPT> L;
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t count = a.counts[cls];
+  const uint32_t* __restrict__ list = a.lists + size_t(cls) * a.n_segments;
+
+  for (uint32_t job = blockIdx.x; job < count; job += gridDim.x) {
+    const uint32_t seg = list[job];
+    const size_t lo = a.offsets[seg];
+    const uint32_t len = uint32_t(seg_end(a, seg) - lo);
+    U* __restrict__ keys = static_cast<U*>(a.keys) + lo;
+    U k[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) k[j] = keys[min(uint32_t(j) * T + tid, len - 1)];
+    to_keys(k, a.mode);
+    const U k0 = to_key(keys[0], a.mode);
+    U vary = 0, mn = k0, mx_k = k0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (uint32_t(j) * T + tid < len) {
+        vary |= U(k[j] ^ k0);
+        mn = k[j] < mn ? k[j] : mn;
+        mx_k = k[j] > mx_k ? k[j] : mx_k;
+      }
+    for (unsigned i = tid; i < unsigned(kMsdBins); i += T) L.cnt[i] = 0;
+    if (tid == 0) {
+      L.flags[0] = 0;
+      L.flags[1] = 0;
+      L.flags[2] = 0;
+      L.kmin = k0;
+      L.kmax = k0;
+    }
+    __syncthreads();
+    {
+      const uint64_t x = wave_or<uint64_t>(uint64_t(vary));
+      uint64_t a0 = uint64_t(mn), a1 = uint64_t(mx_k);
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t o0 = __shfl_xor(a0, d, 64), o1 = __shfl_xor(a1, d, 64);
+        a0 = o0 < a0 ? o0 : a0;
+        a1 = o1 > a1 ? o1 : a1;
+      }
+      if (lane == 0 && x) {
+        atomicOr(&L.flags[0], uint32_t(x));
+        if (P == 8) atomicOr(&L.flags[1], uint32_t(x >> 32));
+        atomicMin(&L.kmin, U(a0));
+        atomicMax(&L.kmax, U(a1));
+      }
+    }
+    __syncthreads();
+    const uint64_t varying = uint64_t(L.flags[0]) | (uint64_t(L.flags[1]) << 32);
+    if (varying == 0) {  // every key equal: the segment is sorted as it is
+      __syncthreads();
+      continue;
+    }
+    // Digit: the top kMsdBits bits of the key's offset from the segment's
+    // smallest key, so the bins split the keys' actual range.
+    const U kmin = L.kmin;
+    const uint64_t span = uint64_t(L.kmax - kmin);
+    const int bits = 64 - __clzll((long long)span);
+    const unsigned sh = bits > kMsdBits ? unsigned(bits - kMsdBits) : 0u;
+    auto dig = [&](U x) { return uint32_t(uint64_t(U(x - kmin)) >> sh); };
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (uint32_t(j) * T + tid < len) atomicAdd(&L.cnt[dig(k[j])], 1u);
+    __syncthreads();
+    // Run starts (digits in order) and the longest run.
+    uint32_t c[PER], sum = 0, mx = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      c[q] = L.cnt[tid * PER + q];
+      sum += c[q];
+      mx = max(mx, c[q]);
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) mx = max(mx, uint32_t(__shfl_xor(mx, d, 64)));
+    if (lane == 0) atomicMax(&L.flags[2], mx);
+    uint32_t run = block_exclusive_add<T>(sum, L.wsum);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      L.cnt[tid * PER + q] = run;
+      run += c[q];
+    }
+    __syncthreads();
+    // Scatter: a key's slot is its digit's next free one (any order inside a run).
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (uint32_t(j) * T + tid < len) L.keys[atomicAdd(&L.cnt[dig(k[j])], 1u)] = k[j];
+    __syncthreads();  // L.cnt[d] is now the end of run d
+    if (L.flags[2] <= uint32_t(kMsdRunMax)) {
+      for (unsigned d = tid; d < unsigned(kMsdBins); d += T) {
+        const uint32_t b = d ? L.cnt[d - 1] : 0u, e = L.cnt[d];
+        for (uint32_t i = b + 1; i < e; ++i) {
+          const U x = L.keys[i];
+          uint32_t j = i;
+          while (j > b && L.keys[j - 1] > x) {
+            L.keys[j] = L.keys[j - 1];
+            --j;
+          }
+          L.keys[j] = x;
+        }
+      }
+    } else {
+      // LSD byte passes over the bytes in which the keys differ; padding
+      // slots hold the largest key and stay last.
+      __syncthreads();  // every lane has read L.cnt and L.flags before whist reuses them
+      for (unsigned i = len + tid; i < unsigned(CAP); i += T) L.keys[i] = U(~U(0));
+      __syncthreads();
+      const uint32_t wb = w * 64 * IPT + lane;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) k[j] = L.keys[wb + 64u * j];
+#pragma unroll 1
+      for (int p = 0; p < P; ++p) {
+        const unsigned shift = 8u * unsigned(p);
+        if (((varying >> shift) & 0xFFu) == 0) continue;
+        for (unsigned x = lane; x < 256; x += 64) L.whist[w][x] = 0;
+        uint32_t rk[(IPT + 1) / 2];
+        if (a.atomic_rank)
+          rank_in_wave_atomic<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
+        else
+          rank_in_wave<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
+        __syncthreads();
+        uint32_t cnt = 0;
+        if (tid < 256) {
+#pragma unroll
+          for (int ww = 0; ww < NW; ++ww) cnt += L.whist[ww][tid];
+        }
+        uint32_t base = block256_exclusive_add(cnt, L.wsum);
+        if (tid < 256) {
+#pragma unroll
+          for (int ww = 0; ww < NW; ++ww) {
+            const uint32_t cc = L.whist[ww][tid];
+            L.whist[ww][tid] = base;
+            base += cc;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          const uint32_t d = digit_of(k[j], shift);
+          L.keys[L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu)] = k[j];
+          if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) k[j] = L.keys[wb + 64u * j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = uint32_t(j) * T + tid;
+      if (i < len) keys[i] = from_key<U>(L.keys[i], a.mode);
+    }
+    __syncthreads();  // the next segment reuses the LDS
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Three-way split of long keys-only segments around a pivot key (equal keys
 // are equal bits, so no order among them is observable): keys below the
 // pivot go to the front of the segment, keys above to the back, and the
@@ -704,6 +906,7 @@ struct ClassShape {
   int bitonic_w;  // > 0: wave network of this width
   int threads, ipt;
   bool bytes;  // byte digits (k_seg_radix8) instead of nibbles
+  bool msd;    // keys only: one most-significant-digit pass and short sorts (k_seg_msd)
 };
 
 // Class c, variant v: variant 0 is the default; the others are the
@@ -723,16 +926,25 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
     case 5:
       return v == 0 ? ClassShape{0, 128, 16, false} : (alt && v == 1 ? ClassShape{0, 256, 8, true} : ClassShape{});
     case 6:
-      return v == 0 ? ClassShape{0, 256, 16, true} : (alt && v == 1 ? ClassShape{0, 256, 16, false} : ClassShape{});
+      return v == 0   ? ClassShape{0, 256, 16, true}
+             : !alt   ? ClassShape{}
+             : v == 1 ? ClassShape{0, 256, 16, false}
+             : v == 2 && !VALS ? ClassShape{0, 256, 16, true, true}
+                               : ClassShape{};
     case 7:
       return v == 0   ? ClassShape{0, 512, 16, true}
              : !alt   ? ClassShape{}
              : v == 1 ? ClassShape{0, 512, 16, false}
              : v == 2 ? ClassShape{0, 1024, 8, true}
-                      : ClassShape{};
+             : v == 3 && !VALS ? ClassShape{0, 512, 16, true, true}
+                               : ClassShape{};
     case 8:
       if (wide && VALS) return ClassShape{};
-      return v == 0 ? ClassShape{0, 1024, 16, true} : (alt && v == 1 ? ClassShape{0, 512, 32, false} : ClassShape{});
+      return v == 0   ? ClassShape{0, 1024, 16, true}
+             : !alt   ? ClassShape{}
+             : v == 1 ? ClassShape{0, 512, 32, false}
+             : v == 2 && !VALS ? ClassShape{0, 1024, 16, true, true}
+                               : ClassShape{};
     default: return ClassShape{};
   }
 }
@@ -759,6 +971,14 @@ void launch_shape(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk
   if constexpr (sh.bitonic_w > 0) {
     if (clk) clk->begin(s, "k_seg_bitonic", 0);
     hipLaunchKernelGGL((k_seg_bitonic<U, sh.bitonic_w, VALS>), dim3(num_cus * 8), dim3(256), 0, s, a, C);
+    if (clk) clk->end(s);
+  } else if constexpr (sh.threads > 0 && sh.msd) {
+    constexpr size_t lds = sizeof(SegMsdLds<U, sh.threads, sh.ipt>);
+    constexpr int by_lds = int((160 * 1024) / lds);
+    constexpr int by_threads = 2048 / sh.threads;
+    constexpr int per_cu = std::max(1, std::min(by_lds, by_threads));
+    if (clk) clk->begin(s, "k_seg_msd", 0);
+    hipLaunchKernelGGL((k_seg_msd<U, sh.threads, sh.ipt>), dim3(num_cus * per_cu), dim3(sh.threads), 0, s, a, C);
     if (clk) clk->end(s);
   } else if constexpr (sh.threads > 0 && sh.bytes) {
     constexpr size_t lds = sizeof(SegByteLds<U, sh.threads, sh.ipt, VALS>);
@@ -787,6 +1007,8 @@ void launch_class(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk
     if constexpr (class_exists<U, VALS>(C, 1)) return launch_shape<U, VALS, C, 1>(a, num_cus, s, clk);
   } else if (v == 2) {
     if constexpr (class_exists<U, VALS>(C, 2)) return launch_shape<U, VALS, C, 2>(a, num_cus, s, clk);
+  } else if (v == 3) {
+    if constexpr (class_exists<U, VALS>(C, 3)) return launch_shape<U, VALS, C, 3>(a, num_cus, s, clk);
   }
   launch_shape<U, VALS, C, 0>(a, num_cus, s, clk);
 }
@@ -808,6 +1030,9 @@ template <class U, bool VALS> const char* variant_name(int c, int v) {
   }
   if (sh.bitonic_w)
     std::snprintf(buf[c][v], sizeof buf[c][v], "bitonic, %d lanes per segment", sh.bitonic_w);
+  else if (sh.msd)
+    std::snprintf(buf[c][v], sizeof buf[c][v], "one %d-bit MSD pass, %d lanes x %d keys", kMsdBits, sh.threads,
+                  sh.ipt);
   else
     std::snprintf(buf[c][v], sizeof buf[c][v], "%s digits, %d lanes x %d keys", sh.bytes ? "byte" : "4-bit",
                   sh.threads, sh.ipt);
