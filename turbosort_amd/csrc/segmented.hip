@@ -552,11 +552,11 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
 // passes need not be stable: the keys (in registers) are counted by the top
 // kMsdBits bits of their offset from the segment's smallest key, the counts
 // scanned, and every key scattered into LDS at its digit's next free slot
-// (an LDS atomic add); each lane then insertion-sorts a few of the 4096 short
-// runs.  A segment with a run longer than kMsdRunMax keys (a cluster of
-// distinct keys: insertion sort would be quadratic in it) is finished by the
-// byte-digit LSD passes of k_seg_radix8 instead, over the bytes in which its
-// keys differ.
+// (an LDS atomic add); every key then finds its place in its short run by
+// counting the run's smaller keys.  A segment with a run longer than
+// kMsdRunMax keys (a cluster of distinct keys: the count is quadratic in the
+// run) is finished by the byte-digit LSD passes of k_seg_radix8 instead, over
+// the bytes in which its keys differ.
 constexpr int kMsdBits = 12, kMsdBins = 1 << kMsdBits, kMsdRunMax = 64;
 
 template <class U, int T, int IPT> struct SegMsdLds {
@@ -680,18 +680,33 @@ __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
       if (uint32_t(j) * T + tid < len) L.keys[atomicAdd(&L.cnt[dig(k[j])], 1u)] = k[j];
     __syncthreads();  // L.cnt[d] is now the end of run d
     if (L.flags[2] <= uint32_t(kMsdRunMax)) {
-      for (unsigned d = tid; d < unsigned(kMsdBins); d += T) {
-        const uint32_t b = d ? L.cnt[d - 1] : 0u, e = L.cnt[d];
-        for (uint32_t i = b + 1; i < e; ++i) {
-          const U x = L.keys[i];
-          uint32_t j = i;
-          while (j > b && L.keys[j - 1] > x) {
-            L.keys[j] = L.keys[j - 1];
-            --j;
+      // Each key's final slot: its run's start plus the keys of the run that
+      // are smaller, or equal and earlier (independent reads, no chains);
+      // then the keys are read and all move at once.
+      uint32_t to[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t p = uint32_t(j) * T + tid;
+        to[j] = p;
+        if (p < len) {
+          const U x = L.keys[p];
+          const uint32_t d = dig(x);
+          const uint32_t b = d ? L.cnt[d - 1] : 0u, e = L.cnt[d];
+          uint32_t r = b;
+          for (uint32_t q = b; q < e; ++q) {
+            const U y = L.keys[q];
+            r += (y < x || (y == x && q < p)) ? 1u : 0u;
           }
-          L.keys[j] = x;
+          to[j] = r;
         }
       }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) k[j] = L.keys[uint32_t(j) * T + tid];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        if (uint32_t(j) * T + tid < len) L.keys[to[j]] = k[j];
     } else {
       // LSD byte passes over the bytes in which the keys differ; padding
       // slots hold the largest key and stay last.
