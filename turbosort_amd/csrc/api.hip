@@ -34,6 +34,10 @@ struct ts_ctx {
   // ts_sort_host: device copy of the caller's host slice (grow-only).
   char* hdev = nullptr;
   size_t hdev_bytes = 0;
+  // ts_learned_sort: a second stream (default priority) on which the
+  // host-driven rounds for overflowing buckets run beside the bucket sort.
+  hipStream_t aux = nullptr;
+  hipEvent_t aux_done = nullptr;
 };
 
 namespace {
@@ -412,6 +416,8 @@ void ts_ctx_destroy(ts_ctx* c) {
   DeviceScope ds(c->device);
   if (c->ws) hipFree(c->ws);
   if (c->hdev) hipFree(c->hdev);
+  if (c->aux_done) hipEventDestroy(c->aux_done);
+  if (c->aux) hipStreamDestroy(c->aux);
   if (c->clk.made)
     for (int i = 0; i < 2 * ts::KernelClock::kMax; ++i) hipEventDestroy(c->clk.ev[i]);
   delete c;
@@ -745,6 +751,7 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   sj.num_cus = c->num_cus;
   sj.atomic_rank = c->atomic_rank;
   set_variants(c, kb, false, sj);
+  sj.stage = 1;  // classify; the buckets are sorted below, once the overflow list is known
   e = ts::segmented_sort(sj, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
 
@@ -771,10 +778,24 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
       fat.emplace_back(lo, m);
     }
   }
+  // The buckets that fit a workgroup are sorted on the caller's stream while
+  // the overflowing ones (disjoint ranges, their own scratch) go through the
+  // split rounds, whose host waits then overlap that sort.
+  sj.stage = 2;
+  e = ts::segmented_sort(sj, s, &c->clk);
+  if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
   uint32_t rounds = 0;
   if (!fat.empty()) {
-    st = sort_long_keys(c, kb, mode, data, j.scratch, n, std::move(fat), w + off[14], fmax, s, &rounds);
+    if (!c->aux) {
+      e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&c->aux_done, hipEventDisableTiming);
+      if (e != hipSuccess) return hip_fail(c, e, "auxiliary stream");
+    }
+    st = sort_long_keys(c, kb, mode, data, j.scratch, n, std::move(fat), w + off[14], fmax, c->aux, &rounds);
     if (st != TS_OK) return st;
+    e = hipEventRecord(c->aux_done, c->aux);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, c->aux_done, 0);
+    if (e != hipSuccess) return hip_fail(c, e, "auxiliary stream join");
   }
   if (report) {
     report->sample = S;

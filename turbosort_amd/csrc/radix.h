@@ -171,6 +171,9 @@ struct SegJob {
   int num_cus = 256;
   uint8_t variant[kSegClasses] = {};  // kernel shape per size class (0: default)
   bool atomic_rank = false;           // as RadixJob::atomic_rank
+  // 0: classify and sort; 1: classify only (counts and lists filled);
+  // 2: sort the classes of an earlier stage-1 call.
+  int stage = 0;
 };
 uint32_t seg_lds_max_len(int key_bytes, bool vals);
 // Kernel shapes a size class can run (variant 0 is the default), and a name

@@ -1067,12 +1067,15 @@ template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStrea
   a.mode = j.mode;
   a.atomic_rank = j.atomic_rank ? 1 : 0;
   for (int c = 0; c < kSegClasses; ++c) a.caps[c] = c < kSegDevice ? class_cap<U, VALS>(c) : 0u;
-  hipLaunchKernelGGL(k_seg_clear, dim3(1), dim3(64), 0, s, j.counts);
-  const size_t per_block = size_t(kClassifyThreads) * kClassifyItems;
-  const unsigned cgrid = unsigned((j.n_segments + per_block - 1) / per_block);
-  if (clk) clk->begin(s, "k_seg_classify", 0);
-  hipLaunchKernelGGL(k_seg_classify, dim3(cgrid), dim3(kClassifyThreads), 0, s, a);
-  if (clk) clk->end(s);
+  if (j.stage != 2) {
+    hipLaunchKernelGGL(k_seg_clear, dim3(1), dim3(64), 0, s, j.counts);
+    const size_t per_block = size_t(kClassifyThreads) * kClassifyItems;
+    const unsigned cgrid = unsigned((j.n_segments + per_block - 1) / per_block);
+    if (clk) clk->begin(s, "k_seg_classify", 0);
+    hipLaunchKernelGGL(k_seg_classify, dim3(cgrid), dim3(kClassifyThreads), 0, s, a);
+    if (clk) clk->end(s);
+  }
+  if (j.stage == 1) return hipGetLastError();
   // Longest segments first, so the short classes fill in behind them.
   launch_class<U, VALS, 8>(a, j.num_cus, s, clk, j.variant[8]);
   launch_class<U, VALS, 7>(a, j.num_cus, s, clk, j.variant[7]);
