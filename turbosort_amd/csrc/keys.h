@@ -223,10 +223,30 @@ __host__ __device__ __forceinline__ void learned_scale(unsigned long long xhi, u
 }
 
 // The model coordinate of a radix key: u in [0, kKnots << kKnotFracBits).
+// x * cmul >> cshift, computed as the high word of (x << (32 - cshift)) *
+// cmul: the same value (x << (32 - cshift) stays below 2^32), one multiply.
 __host__ __device__ __forceinline__ uint32_t learned_coord(unsigned long long key, const LearnedModel& m) {
   unsigned long long x = learned_x(key, m);
   x = x > m.xhi ? m.xhi : x;
-  return uint32_t((unsigned long long)uint32_t(x) * m.cmul >> m.cshift);
+  const uint32_t xs = uint32_t(x) << (32 - m.cshift);
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umulhi(xs, m.cmul);
+#else
+  return uint32_t((unsigned long long)xs * m.cmul >> 32);
+#endif
+}
+
+// The knots as evaluated on the device: entry i packs F at knot i (low word)
+// and its rise to knot i + 1 (high word), so one read serves a key.
+__host__ __device__ __forceinline__ unsigned long long knot_pair(const uint32_t* knots, int i) {
+  return (unsigned long long)knots[i] | (unsigned long long)(knots[i + 1] - knots[i]) << 32;
+}
+
+// learned_cdf over knot_pair entries (the same value).
+__host__ __device__ __forceinline__ uint32_t learned_cdf_pairs(uint32_t u, const unsigned long long* pairs) {
+  const uint32_t i = u >> kKnotFracBits, f = u & ((1u << kKnotFracBits) - 1);
+  const unsigned long long p = pairs[i];
+  return uint32_t(p) + uint32_t((p >> 32) * f >> kKnotFracBits);
 }
 
 __host__ __device__ __forceinline__ uint32_t learned_cdf(uint32_t u, const uint32_t* knots) {

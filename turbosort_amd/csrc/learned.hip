@@ -383,12 +383,12 @@ constexpr int kBoundRun = 8;
 template <class U>
 __global__ void k_learn_bounds(const U* __restrict__ data, size_t n, int mode, const LearnedModel* __restrict__ model,
                                unsigned long long* __restrict__ offsets, uint32_t B) {
-  __shared__ uint32_t knots[kKnots + 1];
-  for (unsigned i = threadIdx.x; i <= unsigned(kKnots); i += blockDim.x) knots[i] = model->knots[i];
+  __shared__ unsigned long long knots[kKnots];
+  for (unsigned i = threadIdx.x; i < unsigned(kKnots); i += blockDim.x) knots[i] = knot_pair(model->knots, int(i));
   __syncthreads();
   const LearnedModel& m = *model;
   auto bucket = [&](U x) {
-    return learned_cdf(learned_coord((unsigned long long)to_key<U>(x, mode), m), knots) >> m.bucket_shift;
+    return learned_cdf_pairs(learned_coord((unsigned long long)to_key<U>(x, mode), m), knots) >> m.bucket_shift;
   };
   const size_t runs = (n + kBoundRun - 1) / kBoundRun;
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;

@@ -91,14 +91,16 @@ template <class U> struct DigitSource<U, true> {
   static constexpr int P = 3;
 };
 
-// Copies the model's knots into LDS (every lane calls; a barrier follows).
-__device__ __forceinline__ void load_knots(uint32_t* lds, const LearnedModel* m) {
-  for (unsigned i = threadIdx.x; i <= unsigned(kKnots); i += blockDim.x) lds[i] = m->knots[i];
+// Copies the model's knots into LDS as knot_pair entries (every lane calls;
+// a barrier follows).
+__device__ __forceinline__ void load_knots(unsigned long long* lds, const LearnedModel* m) {
+  for (unsigned i = threadIdx.x; i < unsigned(kKnots); i += blockDim.x) lds[i] = knot_pair(m->knots, int(i));
 }
 
 template <class U>
-__device__ __forceinline__ uint32_t bucket_of(U k, const uint32_t* knots, const LearnedModel& m, uint32_t shift) {
-  return learned_cdf(learned_coord((unsigned long long)k, m), knots) >> shift;
+__device__ __forceinline__ uint32_t bucket_of(U k, const unsigned long long* knots, const LearnedModel& m,
+                                              uint32_t shift) {
+  return learned_cdf_pairs(learned_coord((unsigned long long)k, m), knots) >> shift;
 }
 
 // Presorted scan (presorted.rs:15-65) on the device: flags whether some
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ key
   // no pass runs, so no counts.
   if (!LEARNED && check_order && (!m->breaks_up || (check_order == 2 && !m->breaks_down))) return;
   __shared__ uint32_t h[P * kSegments * 256];
-  __shared__ uint32_t knots[LEARNED ? kKnots + 1 : 1];
+  __shared__ unsigned long long knots[LEARNED ? kKnots : 1];
   const unsigned tid = threadIdx.x, lane = tid & 63;
   if (LEARNED) load_knots(knots, model);
   for (unsigned i = tid; i < P * kSegments * 256; i += kHistThreads) h[i] = 0;
@@ -391,7 +393,7 @@ template <class U, bool VALS, int IPT, bool LEARNED = false> struct SweepLds {
   U keys[TILE];
   uint32_t vals[VALS ? TILE : 1];
   uint8_t sdig[LEARNED ? TILE : 1];  // staged keys' digits (learned digits are not a key field)
-  uint32_t knots[LEARNED ? kKnots + 1 : 1];
+  unsigned long long knots[LEARNED ? kKnots : 1];
   unsigned long long gofs[256];
   uint32_t wave[NW];
   uint32_t tcnt[256];
