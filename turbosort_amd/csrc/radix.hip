@@ -72,6 +72,9 @@ constexpr int kHistThreads = 256;
 #ifndef TS_HIST_BLOCKS_PER_CU_WIDE
 #define TS_HIST_BLOCKS_PER_CU_WIDE 2
 #endif
+#ifndef TS_HIST_BLOCKS_PER_CU_LEARNED
+#define TS_HIST_BLOCKS_PER_CU_LEARNED 4
+#endif
 // Measured on MI355X at 2^28 keys: 16 keys per lane per step; 8-byte keys
 // (eight digit tables per workgroup) prefer fewer, longer workgroups.
 template <class U> struct HistUnroll {
@@ -794,7 +797,10 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   // contiguous chunk of it.
   const bool vals = j.final_vals != nullptr;
   const size_t seg0 = first_segment(j.n, KB, LEARNED, vals);
-  const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * HistUnroll<U>::blocks_per_cu / kSegments));
+  // The learned histogram holds three digit tables (and the knots), not
+  // eight: more of its workgroups fit a CU.
+  const int hist_per_cu = LEARNED ? TS_HIST_BLOCKS_PER_CU_LEARNED : HistUnroll<U>::blocks_per_cu;
+  const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * hist_per_cu / kSegments));
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
   size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
   per_block = std::max(step, ((per_block + step - 1) / step) * step);
