@@ -379,6 +379,9 @@ __global__ void k_pwl_uniform(LearnedModel* model) {
 // bucket is >= b, offsets[B] = n.  Each lane takes kBoundRun consecutive keys
 // and evaluates the model once per key (plus once for the key before its run).
 constexpr int kBoundRun = 8;
+// Binary searches (k_learn_bounds_search) replace the scan when there are at
+// least this many keys per bucket.
+constexpr size_t kBoundSearchRatio = 64;
 
 template <class U>
 __global__ void k_learn_bounds(const U* __restrict__ data, size_t n, int mode, const LearnedModel* __restrict__ model,
@@ -408,6 +411,35 @@ __global__ void k_learn_bounds(const U* __restrict__ data, size_t n, int mode, c
     if (i0 + kBoundRun >= n)  // this run holds the last key
       for (uint32_t b = prev; b <= B; ++b) offsets[b] = n;
   }
+}
+
+// The same bounds when buckets are few against keys: lane b finds the first
+// position whose bucket is >= b by binary search over the keys (in bucket
+// order), about log2(n) dependent reads per bucket instead of a read of
+// every key.  The searches' upper levels share their positions and hit L2.
+template <class U>
+__global__ __launch_bounds__(256) void k_learn_bounds_search(const U* __restrict__ data, size_t n, int mode,
+                                                             const LearnedModel* __restrict__ model,
+                                                             unsigned long long* __restrict__ offsets, uint32_t B) {
+  __shared__ unsigned long long knots[kKnots];
+  for (unsigned i = threadIdx.x; i < unsigned(kKnots); i += blockDim.x) knots[i] = knot_pair(model->knots, int(i));
+  __syncthreads();
+  const LearnedModel& m = *model;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > B) return;
+  size_t lo = 0, hi = n;  // the answer lies in [lo, hi]
+  if (b == B) lo = n;
+  if (b == 0) hi = 0;
+  while (lo < hi) {
+    const size_t mid = lo + ((hi - lo) >> 1);
+    const uint32_t k = learned_cdf_pairs(learned_coord((unsigned long long)to_key<U>(data[mid], mode), m), knots) >>
+                       m.bucket_shift;
+    if (k < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  offsets[b] = lo;
 }
 
 // Largest bucket: stats[0] = max over b of offsets[b+1] - offsets[b].
@@ -484,10 +516,16 @@ template <class U> hipError_t run_learned(const LearnedJob& j, hipStream_t s, Ke
   p.model = j.model;
   e = radix_sort(p, s, clk);
   if (e != hipSuccess) return e;
-  const unsigned bg = unsigned(std::min<size_t>((j.n / kBoundRun + 256) / 256, size_t(j.num_cus) * 16));
-  if (clk) clk->begin(s, "k_learn_bounds", double(j.n) * sizeof(U));
-  hipLaunchKernelGGL((k_learn_bounds<U>), dim3(bg), dim3(256), 0, s, static_cast<const U*>(j.data), j.n, j.mode,
-                     j.model, j.offsets, j.buckets);
+  if (size_t(j.buckets) * kBoundSearchRatio <= j.n) {
+    if (clk) clk->begin(s, "k_learn_bounds_search", 0);
+    hipLaunchKernelGGL((k_learn_bounds_search<U>), dim3(j.buckets / 256 + 1), dim3(256), 0, s,
+                       static_cast<const U*>(j.data), j.n, j.mode, j.model, j.offsets, j.buckets);
+  } else {
+    const unsigned bg = unsigned(std::min<size_t>((j.n / kBoundRun + 256) / 256, size_t(j.num_cus) * 16));
+    if (clk) clk->begin(s, "k_learn_bounds", double(j.n) * sizeof(U));
+    hipLaunchKernelGGL((k_learn_bounds<U>), dim3(bg), dim3(256), 0, s, static_cast<const U*>(j.data), j.n, j.mode,
+                       j.model, j.offsets, j.buckets);
+  }
   if (clk) clk->end(s);
   hipLaunchKernelGGL(k_bucket_max, dim3(1), dim3(1024), 0, s, j.offsets, j.buckets, j.stats);
   return hipGetLastError();
