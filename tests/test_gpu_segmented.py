@@ -162,6 +162,45 @@ def test_long_keys_only_segments_split(gpu, dtype):
     check(keys, off, None, gpu)
 
 
+@pytest.mark.parametrize("dtype", ["u64", "f32"])
+def test_very_long_keys_only_segments(gpu, dtype):
+    """Keys-only segments of over 64 workgroups' capacity (several split
+    rounds, few distinct values in one of them) beside shorter ones."""
+    cap = ts.segmented_lds_max_len(torch.uint64 if dtype == "u64" else torch.float32, False)
+    big = 64 * cap + 1000
+    g = rng(61)
+    a = random_of(dtype, big, seed=62)
+    b = random_of(dtype, 7, seed=63)[g.integers(0, 7, size=big + 5)]
+    keys = np.concatenate([a, random_of(dtype, 3 * cap, seed=64), b, random_of(dtype, 100, seed=65)])
+    off = offsets_of([big, 3 * cap, big + 5, 100], lead=0)
+    check(keys, off, None, gpu)
+
+
+@pytest.mark.parametrize("with_values", [False, True])
+def test_more_long_segments_than_one_readback(gpu, with_values):
+    """More segments past the LDS capacity than the first bounds readback
+    holds (8192): the rest arrive in a second copy; every segment sorted."""
+    dt = torch.uint8
+    cap = ts.segmented_lds_max_len(dt, with_values)
+    count = 8192 + 37
+    lens = [cap + 1 + (i % 3) for i in range(count)]
+    off = offsets_of(lens, lead=3)
+    n = int(off[-1]) + 2
+    keys = random_of("u8", n, seed=91)
+    vals = np.arange(n, dtype=np.uint32) if with_values else None
+    got_k, got_v = run(keys, off, vals, gpu, 0)
+    assert np.array_equal(got_k[:3], keys[:3]) and np.array_equal(got_k[-2:], keys[-2:])
+    for i in range(count):
+        lo, hi = int(off[i]), int(off[i + 1])
+        seg = keys[lo:hi]
+        if with_values:  # stable: equal keys keep their input order
+            order = np.argsort(seg, kind="stable")
+            assert np.array_equal(got_k[lo:hi], seg[order])
+            assert np.array_equal(got_v[lo:hi], vals[lo:hi][order])
+        else:
+            assert np.array_equal(got_k[lo:hi], np.sort(seg))
+
+
 def test_graph_capture_replays(gpu):
     """With a bound that fits one workgroup the sort makes no host call, so
     it can be captured in a graph; every replay sorts fresh input exactly

@@ -38,6 +38,11 @@ struct ts_ctx {
   // host-driven rounds for overflowing buckets run beside the bucket sort.
   hipStream_t aux = nullptr;
   hipEvent_t aux_done = nullptr;
+  hipEvent_t back_ready = nullptr;  // the learned sort's readback has landed
+  // Pinned host memory for readbacks and the split rounds' segment lists
+  // (grow-only), so those copies are direct transfers.
+  char* pin = nullptr;
+  size_t pin_bytes = 0;
 };
 
 namespace {
@@ -81,6 +86,26 @@ ts_status ensure(ts_ctx* c, size_t bytes) {
     return hip_fail(c, e, "hipMalloc(workspace)");
   }
   c->ws_bytes = bytes;
+  return TS_OK;
+}
+
+// Grows the pinned host area; copies from an earlier call may still read
+// the old one, so the device is drained before it is freed.
+ts_status ensure_pinned(ts_ctx* c, size_t bytes) {
+  if (bytes <= c->pin_bytes) return TS_OK;
+  if (c->pin) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipHostFree(c->pin);
+    c->pin = nullptr;
+    c->pin_bytes = 0;
+    if (e != hipSuccess) return hip_fail(c, e, "hipHostFree");
+  }
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&c->pin), bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    c->pin = nullptr;
+    return hip_fail(c, e, "hipHostMalloc(staging)");
+  }
+  c->pin_bytes = bytes;
   return TS_OK;
 }
 
@@ -215,20 +240,71 @@ size_t long_keys_bytes(size_t f) {
   return 5 * up(f * 8) + up(f * 4) + 2 * up(2 * f * 8) + up((ts::kSegClasses + 2) * 4) +
          up(size_t(ts::kSegClasses) * 2 * f * 4);
 }
+// Pinned host arrays for sort_long_keys: a round's starts and lengths, chunk
+// starts, split counts, and the parts' starts and ends.
+size_t long_keys_pin_bytes(size_t f) { return 2 * up(f * 8) + up(f * 4) + up(2 * f * 8) + 2 * up(2 * f * 8); }
 // At most this many segments longer than `cap` fit in n keys.
 size_t long_keys_max(size_t n, size_t cap) { return n / (cap + 1) + 1; }
+
+// Reading a seg_long_bounds block (room for fmax pairs) into the pinned
+// area, whose start then holds the header: start_long_bounds queues a copy of
+// the header and the first kReadbackPairs pairs; once it is complete,
+// finish_long_bounds copies the rest (if any) on stream s2, waits for it and
+// lists the long segments as (start, length).
+constexpr size_t kReadbackPairs = 8192;
+hipError_t start_long_bounds(ts_ctx* c, const unsigned long long* back, size_t fmax, hipStream_t s) {
+  return hipMemcpyAsync(c->pin, back, ts::kSegLongHeader + std::min(fmax, kReadbackPairs) * 16,
+                        hipMemcpyDeviceToHost, s);
+}
+hipError_t finish_long_bounds(ts_ctx* c, const unsigned long long* back, size_t fmax, hipStream_t s2,
+                              std::vector<std::pair<uint64_t, uint64_t>>* fat) {
+  const size_t first = std::min(fmax, kReadbackPairs);
+  const size_t nbig = std::min<size_t>(reinterpret_cast<const uint32_t*>(c->pin)[ts::kSegDevice], fmax);
+  auto* pairs = reinterpret_cast<const unsigned long long*>(c->pin + ts::kSegLongHeader);
+  if (nbig > first) {
+    hipError_t e = hipMemcpyAsync(c->pin + ts::kSegLongHeader + first * 16, back + ts::kSegLongHeader / 8 + 2 * first,
+                                  (nbig - first) * 16, hipMemcpyDeviceToHost, s2);
+    if (e == hipSuccess) e = hipStreamSynchronize(s2);
+    if (e != hipSuccess) return e;
+  }
+  fat->clear();
+  fat->reserve(nbig);
+  for (size_t i = 0; i < nbig; ++i) fat->emplace_back(pairs[2 * i], pairs[2 * i + 1]);
+  return hipSuccess;
+}
+
+// Device area for the radix sorts of sort_long_keys: meta and look-back
+// status for up to n keys.
+size_t long_radix_bytes(size_t n, int kb) { return up(sizeof(ts::SortMeta)) + up(ts::status_bytes(n, kb)); }
 
 // Keys-only segments too long for one workgroup (equal keys are equal bits,
 // so no order among them is observable): rounds of three-way splits around a
 // pivot; the parts that fit one workgroup go to the segmented LDS classes,
-// longer parts are split again.  After kRounds rounds the rest take the
-// device radix sort.  `tmp` holds n keys; `area` has long_keys_bytes(f) bytes
-// with f >= long_keys_max(n, cap).  Returns the rounds used in *rounds.
+// longer parts are split again (all of a round's segments in the same
+// launches).  What is left after kRounds rounds takes the device radix sort,
+// with the segment's own stretch of `tmp` as its second buffer.  `tmp` holds n keys; `area` has long_keys_bytes(f) bytes,
+// `radix_ws` long_radix_bytes(n, kb) bytes and `pin` long_keys_pin_bytes(f)
+// bytes of pinned host memory, with f >= long_keys_max(n, cap).  Nothing else
+// of the context's workspace is touched, so the caller may still be using
+// it.  Returns the rounds used in *rounds.
 ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, size_t n,
-                         std::vector<std::pair<uint64_t, uint64_t>> segs, char* area, size_t f, hipStream_t s,
-                         uint32_t* rounds) {
+                         std::vector<std::pair<uint64_t, uint64_t>> segs, char* area, char* radix_ws, char* pin,
+                         size_t f, hipStream_t s, uint32_t* rounds) {
   constexpr int kRounds = 12;
   const uint32_t cap = ts::seg_lds_max_len(kb, false);
+  auto radix_part = [&](uint64_t lo, uint64_t len) {
+    ts::RadixJob j;
+    j.key_bytes = kb;
+    j.mode = mode;
+    j.n = len;
+    j.meta = reinterpret_cast<ts::SortMeta*>(radix_ws);
+    j.status = radix_ws + up(sizeof(ts::SortMeta));
+    j.keys_a = static_cast<unsigned char*>(data) + lo * size_t(kb);
+    j.keys_b = static_cast<unsigned char*>(tmp) + lo * size_t(kb);
+    j.num_cus = c->num_cus;
+    j.atomic_rank = c->atomic_rank;
+    return ts::radix_sort(j, s, &c->clk);
+  };
   size_t o = 0;
   auto region = [&](size_t bytes) {
     char* r = area + o;
@@ -245,25 +321,35 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
   auto* d_ends = reinterpret_cast<unsigned long long*>(region(2 * f * 8));
   auto* d_counts = reinterpret_cast<uint32_t*>(region((ts::kSegClasses + 2) * 4));
   auto* d_lists = reinterpret_cast<uint32_t*>(region(size_t(ts::kSegClasses) * 2 * f * 4));
+  // Host side: each array is rewritten only after a stream synchronisation
+  // that follows every copy reading it.
+  size_t po = 0;
+  auto pregion = [&](size_t bytes) {
+    char* r = pin + po;
+    po += up(bytes);
+    return r;
+  };
+  auto* h_lo = reinterpret_cast<unsigned long long*>(pregion(f * 8));
+  auto* h_len = reinterpret_cast<unsigned long long*>(pregion(f * 8));
+  auto* h_chunks = reinterpret_cast<uint32_t*>(pregion(f * 4));
+  auto* h_ba = reinterpret_cast<unsigned long long*>(pregion(2 * f * 8));
+  auto* h_starts = reinterpret_cast<unsigned long long*>(pregion(2 * f * 8));
+  auto* h_ends = reinterpret_cast<unsigned long long*>(pregion(2 * f * 8));
   const uint32_t chunk = ts::split_chunk();
-  std::vector<std::vector<unsigned long long>> keep;  // host sources of the copies, alive until the end
   uint32_t r = 0;
   for (; r < uint32_t(kRounds) && !segs.empty(); ++r) {
     const size_t ns = segs.size();
     if (ns > f) return fail(c, TS_ERR_INVALID_ARG, "internal: too many long segments");
-    keep.emplace_back(2 * ns);
-    std::vector<unsigned long long>& lolen = keep.back();
-    std::vector<uint32_t> cs(ns);
     uint32_t chunks = 0;
     for (size_t i = 0; i < ns; ++i) {
-      lolen[i] = segs[i].first;
-      lolen[ns + i] = segs[i].second;
-      cs[i] = chunks;
+      h_lo[i] = segs[i].first;
+      h_len[i] = segs[i].second;
+      h_chunks[i] = chunks;
       chunks += uint32_t((segs[i].second + chunk - 1) / chunk);
     }
-    hipError_t e = hipMemcpyAsync(d_lo, lolen.data(), ns * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_len, lolen.data() + ns, ns * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_chunks, cs.data(), ns * 4, hipMemcpyHostToDevice, s);
+    hipError_t e = hipMemcpyAsync(d_lo, h_lo, ns * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_len, h_len, ns * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_chunks, h_chunks, ns * 4, hipMemcpyHostToDevice, s);
     ts::SplitJob sj;
     sj.key_bytes = kb;
     sj.mode = mode;
@@ -278,33 +364,29 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
     sj.above = d_above;
     sj.pivot = d_pivot;
     if (e == hipSuccess) e = ts::split_segments(sj, s);
-    std::vector<unsigned long long> ba(2 * ns);
-    if (e == hipSuccess) e = hipMemcpyAsync(ba.data(), d_below, ns * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(ba.data() + ns, d_above, ns * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_ba, d_below, ns * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_ba + ns, d_above, ns * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "split");
     std::vector<std::pair<uint64_t, uint64_t>> next;
-    keep.emplace_back();
-    std::vector<unsigned long long>& se = keep.back();
-    std::vector<unsigned long long> ends;
+    size_t m = 0;
     for (size_t i = 0; i < ns; ++i) {
-      const uint64_t lo = segs[i].first, len = segs[i].second, nb = ba[i], na = ba[ns + i];
+      const uint64_t lo = segs[i].first, len = segs[i].second, nb = h_ba[i], na = h_ba[ns + i];
       const std::pair<uint64_t, uint64_t> parts[2] = {{lo, nb}, {lo + len - na, na}};
       for (const auto& pt : parts) {
         if (pt.second < 2) continue;
         if (pt.second <= cap) {
-          se.push_back(pt.first);
-          ends.push_back(pt.first + pt.second);
+          h_starts[m] = pt.first;
+          h_ends[m] = pt.first + pt.second;
+          ++m;
         } else {
           next.push_back(pt);
         }
       }
     }
-    if (!se.empty()) {
-      const size_t m = se.size();
-      se.insert(se.end(), ends.begin(), ends.end());
-      e = hipMemcpyAsync(d_starts, se.data(), m * 8, hipMemcpyHostToDevice, s);
-      if (e == hipSuccess) e = hipMemcpyAsync(d_ends, se.data() + m, m * 8, hipMemcpyHostToDevice, s);
+    if (m) {
+      e = hipMemcpyAsync(d_starts, h_starts, m * 8, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(d_ends, h_ends, m * 8, hipMemcpyHostToDevice, s);
       ts::SegJob gj;
       gj.key_bytes = kb;
       gj.mode = mode;
@@ -325,12 +407,12 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
     segs.swap(next);
   }
   if (rounds) *rounds = r;
-  for (const auto& sg : segs) {
-    ts_status st = seg_device_sort(c, kb, mode, static_cast<unsigned char*>(data) + sg.first * size_t(kb), nullptr,
-                                   size_t(sg.second), s);
-    if (st != TS_OK) return st;
-  }
-  return TS_OK;
+  hipError_t e = hipSuccess;
+  for (const auto& sg : segs)
+    if (e == hipSuccess) e = radix_part(sg.first, sg.second);
+  // The last parts' copies read pinned memory that the next call may rewrite.
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "split rounds");
 }
 
 }  // namespace
@@ -416,7 +498,9 @@ void ts_ctx_destroy(ts_ctx* c) {
   DeviceScope ds(c->device);
   if (c->ws) hipFree(c->ws);
   if (c->hdev) hipFree(c->hdev);
+  if (c->pin) hipHostFree(c->pin);
   if (c->aux_done) hipEventDestroy(c->aux_done);
+  if (c->back_ready) hipEventDestroy(c->back_ready);
   if (c->aux) hipStreamDestroy(c->aux);
   if (c->clk.made)
     for (int i = 0; i < 2 * ts::KernelClock::kMax; ++i) hipEventDestroy(c->clk.ev[i]);
@@ -597,7 +681,11 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
   DeviceScope ds(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t count_bytes = up((ts::kSegClasses + 2) * sizeof(uint32_t));
-  ts_status st = ensure(c, count_bytes + up(size_t(ts::kSegClasses) * n_segments * sizeof(uint32_t)));
+  const size_t list_bytes = up(size_t(ts::kSegClasses) * n_segments * sizeof(uint32_t));
+  const uint32_t cap = ts::seg_lds_max_len(kb, vals != nullptr);
+  const size_t fmax = long_keys_max(n, cap);  // valid offsets are non-decreasing: segments are disjoint
+  const size_t back_bytes = ts::kSegLongHeader + fmax * 16;
+  ts_status st = ensure(c, count_bytes + list_bytes + up(back_bytes));
   if (st != TS_OK) return st;
   ts::SegJob j;
   j.key_bytes = kb;
@@ -616,35 +704,32 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
   if (max_segment_len != 0 && max_segment_len <= ts::seg_lds_max_len(kb, vals != nullptr)) return TS_OK;
 
-  // Unknown or long segments: read the class counts back once.
-  uint32_t h[ts::kSegClasses + 2];
-  e = hipMemcpyAsync(h, j.counts, sizeof h, hipMemcpyDeviceToHost, s);
+  // Unknown or long segments: read the class counts and the long segments'
+  // bounds back (one copy unless they are many).
+  st = ensure_pinned(c, up(back_bytes) + (vals ? 0 : long_keys_pin_bytes(fmax)));
+  if (st != TS_OK) return st;
+  auto* back = reinterpret_cast<unsigned long long*>(c->ws + count_bytes + list_bytes);
+  std::vector<std::pair<uint64_t, uint64_t>> fat;
+  e = ts::seg_long_bounds(j, back, uint32_t(fmax), nullptr, 0, s);
+  if (e == hipSuccess) e = start_long_bounds(c, back, fmax, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hip_fail(c, e, "segment classes");
-  if (h[ts::kSegClasses]) return fail(c, TS_ERR_INVALID_ARG, "segment offsets must be non-decreasing and at most n");
-  const uint32_t nbig = h[ts::kSegDevice];
-  if (nbig == 0) return TS_OK;
-  std::vector<uint32_t> ids(nbig);
-  e = hipMemcpy(ids.data(), j.lists + size_t(ts::kSegDevice) * n_segments, nbig * sizeof(uint32_t),
-                hipMemcpyDeviceToHost);
-  std::vector<uint64_t> bounds(2 * size_t(nbig));
-  for (uint32_t i = 0; i < nbig && e == hipSuccess; ++i)
-    e = hipMemcpy(&bounds[2 * size_t(i)], offsets + ids[i], 2 * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = finish_long_bounds(c, back, fmax, s, &fat);
   if (e != hipSuccess) return hip_fail(c, e, "segment bounds");
+  if (reinterpret_cast<const uint32_t*>(c->pin)[ts::kSegClasses])
+    return fail(c, TS_ERR_INVALID_ARG, "segment offsets must be non-decreasing and at most n");
+  if (fat.empty()) return TS_OK;
   if (!vals) {
     // Keys only: three-way splits, then the LDS classes (the workspace's
     // class lists are no longer needed, so it may be regrown).
-    const size_t fmax = long_keys_max(n, ts::seg_lds_max_len(kb, false));
-    st = ensure(c, up(n * size_t(kb)) + long_keys_bytes(fmax));
+    st = ensure(c, up(n * size_t(kb)) + long_keys_bytes(fmax) + long_radix_bytes(n, kb));
     if (st != TS_OK) return st;
-    std::vector<std::pair<uint64_t, uint64_t>> fat;
-    for (uint32_t i = 0; i < nbig; ++i) fat.emplace_back(bounds[2 * size_t(i)], bounds[2 * size_t(i) + 1] - bounds[2 * size_t(i)]);
-    return sort_long_keys(c, kb, mode, keys, c->ws, n, std::move(fat), c->ws + up(n * size_t(kb)), fmax, s, nullptr);
+    char* area = c->ws + up(n * size_t(kb));
+    return sort_long_keys(c, kb, mode, keys, c->ws, n, std::move(fat), area, area + long_keys_bytes(fmax),
+                          c->pin + up(back_bytes), fmax, s, nullptr);
   }
-  for (uint32_t i = 0; i < nbig; ++i) {
-    const uint64_t lo = bounds[2 * size_t(i)], m = bounds[2 * size_t(i) + 1] - lo;
-    st = seg_device_sort(c, kb, mode, static_cast<unsigned char*>(keys) + lo * size_t(kb), vals ? vals + lo : nullptr,
-                         size_t(m), s);
+  for (const auto& sg : fat) {
+    st = seg_device_sort(c, kb, mode, static_cast<unsigned char*>(keys) + sg.first * size_t(kb),
+                         vals + sg.first, size_t(sg.second), s);
     if (st != TS_OK) return st;
   }
   return TS_OK;
@@ -687,7 +772,7 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   // | long-bucket arrays
   const size_t cap = ts::seg_lds_max_len(kb, false);
   const size_t fmax = long_keys_max(n, cap);
-  size_t off[15];
+  size_t off[17];
   size_t b = 0;
   auto take = [&](int i, size_t bytes) { off[i] = b; b += up(bytes); };
   take(0, sizeof(ts::SortMeta));
@@ -705,7 +790,11 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   take(12, (ts::kSegClasses + 2) * 4);
   take(13, size_t(ts::kSegClasses) * B * 4);
   take(14, long_keys_bytes(fmax));
+  const size_t back_bytes = ts::kSegLongHeader + fmax * 16;
+  take(15, back_bytes);
+  take(16, long_radix_bytes(n, kb));
   ts_status st = ensure(c, b);
+  if (st == TS_OK) st = ensure_pinned(c, up(back_bytes) + long_keys_pin_bytes(fmax));
   if (st != TS_OK) return st;
   char* w = c->ws;
   ts::LearnedJob j;
@@ -751,47 +840,46 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   sj.num_cus = c->num_cus;
   sj.atomic_rank = c->atomic_rank;
   set_variants(c, kb, false, sj);
-  sj.stage = 1;  // classify; the buckets are sorted below, once the overflow list is known
+  sj.stage = 1;  // classify only: the readback block is queued between the classify and the bucket sort
   e = ts::segmented_sort(sj, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
 
-  uint32_t h[ts::kSegClasses + 2];
-  float loss[3] = {0.f, 0.f, 0.f};
-  unsigned long long largest = 0;
-  e = hipMemcpyAsync(h, sj.counts, sizeof h, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && j.train) e = hipMemcpyAsync(loss, j.loss, sizeof loss, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(&largest, j.stats, sizeof largest, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hip_fail(c, e, "learned buckets");
-  const uint32_t nbig = h[ts::kSegDevice];
+  // One readback: the class counts, the fit's losses (4 words from j.loss)
+  // and the largest bucket (the 2 words of j.stats after them) in the
+  // header, then the overflowing buckets' bounds.
   std::vector<std::pair<uint64_t, uint64_t>> fat;
-  uint64_t okeys = 0;
-  if (nbig) {
-    std::vector<uint32_t> ids(nbig);
-    std::vector<unsigned long long> offs(size_t(B) + 1);
-    e = hipMemcpy(ids.data(), sj.lists + size_t(ts::kSegDevice) * B, nbig * sizeof(uint32_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(offs.data(), j.offsets, offs.size() * 8, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return hip_fail(c, e, "overflow buckets");
-    for (uint32_t i = 0; i < nbig; ++i) {
-      const uint64_t lo = offs[ids[i]], m = offs[ids[i] + 1] - lo;
-      okeys += m;
-      fat.emplace_back(lo, m);
-    }
-  }
-  // The buckets that fit a workgroup are sorted on the caller's stream while
-  // the overflowing ones (disjoint ranges, their own scratch) go through the
-  // split rounds, whose host waits then overlap that sort.
+  auto* back = reinterpret_cast<unsigned long long*>(w + off[15]);
+  // The buckets that fit a workgroup are sorted on the caller's stream
+  // (queued before the host waits for the readback) while the overflowing
+  // ones (disjoint ranges, their own scratch) go through the split rounds on
+  // the second stream, whose host waits then overlap that sort.
+  if (!c->aux) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+  if (e == hipSuccess && !c->aux_done) e = hipEventCreateWithFlags(&c->aux_done, hipEventDisableTiming);
+  if (e == hipSuccess && !c->back_ready) e = hipEventCreateWithFlags(&c->back_ready, hipEventDisableTiming);
+  if (e != hipSuccess) return hip_fail(c, e, "auxiliary stream");
+  e = ts::seg_long_bounds(sj, back, uint32_t(fmax), reinterpret_cast<const uint32_t*>(j.loss), 6, s);
+  if (e == hipSuccess) e = start_long_bounds(c, back, fmax, s);
+  if (e == hipSuccess) e = hipEventRecord(c->back_ready, s);
+  if (e != hipSuccess) return hip_fail(c, e, "learned buckets");
   sj.stage = 2;
   e = ts::segmented_sort(sj, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
+  e = hipEventSynchronize(c->back_ready);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->aux, c->back_ready, 0);
+  if (e == hipSuccess) e = finish_long_bounds(c, back, fmax, c->aux, &fat);
+  if (e != hipSuccess) return hip_fail(c, e, "learned buckets");
+  const uint32_t* h = reinterpret_cast<const uint32_t*>(c->pin);
+  const uint32_t nbig = h[ts::kSegDevice];
+  float loss[3] = {0.f, 0.f, 0.f};
+  unsigned long long largest = 0;
+  if (j.train) std::memcpy(loss, h + ts::kSegClasses + 2, sizeof loss);
+  std::memcpy(&largest, h + ts::kSegClasses + 2 + 4, sizeof largest);
+  uint64_t okeys = 0;
+  for (const auto& sg : fat) okeys += sg.second;
   uint32_t rounds = 0;
   if (!fat.empty()) {
-    if (!c->aux) {
-      e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
-      if (e == hipSuccess) e = hipEventCreateWithFlags(&c->aux_done, hipEventDisableTiming);
-      if (e != hipSuccess) return hip_fail(c, e, "auxiliary stream");
-    }
-    st = sort_long_keys(c, kb, mode, data, j.scratch, n, std::move(fat), w + off[14], fmax, c->aux, &rounds);
+    st = sort_long_keys(c, kb, mode, data, j.scratch, n, std::move(fat), w + off[14], w + off[16],
+                        c->pin + up(back_bytes), fmax, c->aux, &rounds);
     if (st != TS_OK) return st;
     e = hipEventRecord(c->aux_done, c->aux);
     if (e == hipSuccess) e = hipStreamWaitEvent(s, c->aux_done, 0);

@@ -204,6 +204,14 @@ uint32_t split_chunk();
 hipError_t split_segments(const SplitJob& job, hipStream_t s);
 hipError_t segmented_sort(const SegJob& job, hipStream_t s, KernelClock* clk);
 
+// After a classify (stage 0 or 1), one block for the host to read back: a
+// kSegLongHeader-byte header holding counts[0 .. kSegClasses + 1], then
+// extra_words words copied from `extra`; then (start, length) of the first
+// min(counts[kSegDevice], max_pairs) segments of the device class.
+constexpr size_t kSegLongHeader = 128;
+hipError_t seg_long_bounds(const SegJob& job, unsigned long long* out, uint32_t max_pairs, const uint32_t* extra,
+                           uint32_t extra_words, hipStream_t s);
+
 // Learned partition (learned.hip): sample, fit, then group the keys of `data`
 // by model bucket in place and write the bucket bounds.  Every buffer is
 // caller-provided device memory.

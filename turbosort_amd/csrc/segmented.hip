@@ -895,6 +895,25 @@ template <class U> __global__ __launch_bounds__(kSplitThreads) void k_split_fill
   }
 }
 
+// The device class's segment bounds and the class counts in one block
+// (seg_long_bounds).
+__global__ void k_seg_long_bounds(SegArgs a, unsigned long long* out, uint32_t max_pairs, const uint32_t* extra,
+                                  uint32_t extra_words) {
+  uint32_t* hdr = reinterpret_cast<uint32_t*>(out);
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < uint32_t(kSegClasses + 2)) hdr[g] = a.counts[g];
+  if (g < extra_words) hdr[kSegClasses + 2 + g] = extra[g];
+  const uint32_t nbig = min(a.counts[kSegDevice], max_pairs);
+  const uint32_t* __restrict__ list = a.lists + size_t(kSegDevice) * a.n_segments;
+  unsigned long long* pairs = out + kSegLongHeader / 8;
+  for (uint32_t i = g; i < nbig; i += gridDim.x * blockDim.x) {
+    const uint32_t seg = list[i];
+    const unsigned long long lo = a.offsets[seg];
+    pairs[2 * size_t(i)] = lo;
+    pairs[2 * size_t(i) + 1] = seg_end(a, seg) - lo;
+  }
+}
+
 template <class U> hipError_t run_split(const SplitJob& j, hipStream_t s) {
   SplitArgs a;
   a.data = j.data;
@@ -1146,6 +1165,21 @@ const char* seg_variant_name(int key_bytes, bool vals, int size_class, int varia
 }
 
 uint32_t split_chunk() { return kSplitChunk; }
+
+hipError_t seg_long_bounds(const SegJob& j, unsigned long long* out, uint32_t max_pairs, const uint32_t* extra,
+                           uint32_t extra_words, hipStream_t s) {
+  if (kSegLongHeader < (kSegClasses + 2 + extra_words) * 4) return hipErrorInvalidValue;
+  SegArgs a = {};
+  a.offsets = j.offsets;
+  a.ends = j.ends;
+  a.n = j.n;
+  a.n_segments = j.n_segments;
+  a.counts = j.counts;
+  a.lists = j.lists;
+  const unsigned grid = unsigned(std::min<size_t>(std::max<size_t>((size_t(max_pairs) + 255) / 256, 1), 1024));
+  hipLaunchKernelGGL(k_seg_long_bounds, dim3(grid), dim3(256), 0, s, a, out, max_pairs, extra, extra_words);
+  return hipGetLastError();
+}
 
 hipError_t split_segments(const SplitJob& j, hipStream_t s) {
   if (j.nseg == 0) return hipSuccess;
