@@ -237,12 +237,12 @@ void set_variants(const ts_ctx* c, int kb, bool vals, ts::SegJob& j) {
 
 // Device arrays for sort_long_keys: room for `f` long segments per round.
 size_t long_keys_bytes(size_t f) {
-  return 5 * up(f * 8) + up(f * 4) + 2 * up(2 * f * 8) + up((ts::kSegClasses + 2) * 4) +
+  return up(f * 20) + up(f * 16) + up(f * 8) + up(f * 32) + up((ts::kSegClasses + 2) * 4) +
          up(size_t(ts::kSegClasses) * 2 * f * 4);
 }
-// Pinned host arrays for sort_long_keys: a round's starts and lengths, chunk
-// starts, split counts, and the parts' starts and ends.
-size_t long_keys_pin_bytes(size_t f) { return 2 * up(f * 8) + up(f * 4) + up(2 * f * 8) + 2 * up(2 * f * 8); }
+// Pinned host arrays for sort_long_keys: a round's segments (starts,
+// lengths, chunk starts), split counts, and the parts' starts and ends.
+size_t long_keys_pin_bytes(size_t f) { return up(f * 20) + up(f * 16) + up(f * 32); }
 // At most this many segments longer than `cap` fit in n keys.
 size_t long_keys_max(size_t n, size_t cap) { return n / (cap + 1) + 1; }
 
@@ -311,14 +311,13 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
     o += up(bytes);
     return r;
   };
-  auto* d_lo = reinterpret_cast<unsigned long long*>(region(f * 8));
-  auto* d_len = reinterpret_cast<unsigned long long*>(region(f * 8));
-  auto* d_below = reinterpret_cast<unsigned long long*>(region(f * 8));
-  auto* d_above = reinterpret_cast<unsigned long long*>(region(f * 8));
+  // Per round, packed so that each direction takes one copy: the segments
+  // (starts[ns] | lengths[ns] | chunk starts[ns] as u32), the split counts
+  // (below[ns] | above[ns]) and the parts (starts[m] | ends[m], m <= 2 ns).
+  char* d_seg = region(f * 20);
+  auto* d_ba = reinterpret_cast<unsigned long long*>(region(f * 16));
   auto* d_pivot = reinterpret_cast<unsigned long long*>(region(f * 8));
-  auto* d_chunks = reinterpret_cast<uint32_t*>(region(f * 4));
-  auto* d_starts = reinterpret_cast<unsigned long long*>(region(2 * f * 8));
-  auto* d_ends = reinterpret_cast<unsigned long long*>(region(2 * f * 8));
+  auto* d_se = reinterpret_cast<unsigned long long*>(region(f * 32));
   auto* d_counts = reinterpret_cast<uint32_t*>(region((ts::kSegClasses + 2) * 4));
   auto* d_lists = reinterpret_cast<uint32_t*>(region(size_t(ts::kSegClasses) * 2 * f * 4));
   // Host side: each array is rewritten only after a stream synchronisation
@@ -329,27 +328,26 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
     po += up(bytes);
     return r;
   };
-  auto* h_lo = reinterpret_cast<unsigned long long*>(pregion(f * 8));
-  auto* h_len = reinterpret_cast<unsigned long long*>(pregion(f * 8));
-  auto* h_chunks = reinterpret_cast<uint32_t*>(pregion(f * 4));
-  auto* h_ba = reinterpret_cast<unsigned long long*>(pregion(2 * f * 8));
-  auto* h_starts = reinterpret_cast<unsigned long long*>(pregion(2 * f * 8));
-  auto* h_ends = reinterpret_cast<unsigned long long*>(pregion(2 * f * 8));
+  char* h_seg = pregion(f * 20);
+  auto* h_ba = reinterpret_cast<unsigned long long*>(pregion(f * 16));
+  auto* h_se = reinterpret_cast<unsigned long long*>(pregion(f * 32));
   const uint32_t chunk = ts::split_chunk();
+  uint32_t class_cap[ts::kSegDevice];
+  for (int k = 0; k < ts::kSegDevice; ++k) class_cap[k] = ts::seg_class_cap(kb, false, k);
   uint32_t r = 0;
   for (; r < uint32_t(kRounds) && !segs.empty(); ++r) {
     const size_t ns = segs.size();
     if (ns > f) return fail(c, TS_ERR_INVALID_ARG, "internal: too many long segments");
+    auto* h_lo = reinterpret_cast<unsigned long long*>(h_seg);
+    auto* h_chunks = reinterpret_cast<uint32_t*>(h_seg + ns * 16);
     uint32_t chunks = 0;
     for (size_t i = 0; i < ns; ++i) {
       h_lo[i] = segs[i].first;
-      h_len[i] = segs[i].second;
+      h_lo[ns + i] = segs[i].second;
       h_chunks[i] = chunks;
       chunks += uint32_t((segs[i].second + chunk - 1) / chunk);
     }
-    hipError_t e = hipMemcpyAsync(d_lo, h_lo, ns * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_len, h_len, ns * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_chunks, h_chunks, ns * 4, hipMemcpyHostToDevice, s);
+    hipError_t e = hipMemcpyAsync(d_seg, h_seg, ns * 20, hipMemcpyHostToDevice, s);
     ts::SplitJob sj;
     sj.key_bytes = kb;
     sj.mode = mode;
@@ -357,44 +355,49 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
     sj.tmp = tmp;
     sj.nseg = uint32_t(ns);
     sj.chunks = chunks;
-    sj.lo = d_lo;
-    sj.len = d_len;
-    sj.chunk_start = d_chunks;
-    sj.below = d_below;
-    sj.above = d_above;
+    sj.lo = reinterpret_cast<const unsigned long long*>(d_seg);
+    sj.len = reinterpret_cast<const unsigned long long*>(d_seg) + ns;
+    sj.chunk_start = reinterpret_cast<const uint32_t*>(d_seg + ns * 16);
+    sj.below = d_ba;
+    sj.above = d_ba + ns;
     sj.pivot = d_pivot;
     if (e == hipSuccess) e = ts::split_segments(sj, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(h_ba, d_below, ns * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(h_ba + ns, d_above, ns * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_ba, d_ba, ns * 16, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "split");
-    std::vector<std::pair<uint64_t, uint64_t>> next;
-    size_t m = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> next, fit;
+    uint32_t mask = 0;  // size classes the fitting parts fall in
     for (size_t i = 0; i < ns; ++i) {
       const uint64_t lo = segs[i].first, len = segs[i].second, nb = h_ba[i], na = h_ba[ns + i];
       const std::pair<uint64_t, uint64_t> parts[2] = {{lo, nb}, {lo + len - na, na}};
       for (const auto& pt : parts) {
         if (pt.second < 2) continue;
         if (pt.second <= cap) {
-          h_starts[m] = pt.first;
-          h_ends[m] = pt.first + pt.second;
-          ++m;
+          fit.push_back(pt);
+          int k = 0;
+          while (k + 1 < ts::kSegDevice && (class_cap[k] == 0 || pt.second > class_cap[k])) ++k;
+          mask |= 1u << k;
         } else {
           next.push_back(pt);
         }
       }
     }
-    if (m) {
-      e = hipMemcpyAsync(d_starts, h_starts, m * 8, hipMemcpyHostToDevice, s);
-      if (e == hipSuccess) e = hipMemcpyAsync(d_ends, h_ends, m * 8, hipMemcpyHostToDevice, s);
+    if (!fit.empty()) {
+      const size_t m = fit.size();
+      for (size_t i = 0; i < m; ++i) {
+        h_se[i] = fit[i].first;
+        h_se[m + i] = fit[i].first + fit[i].second;
+      }
+      e = hipMemcpyAsync(d_se, h_se, m * 16, hipMemcpyHostToDevice, s);
       ts::SegJob gj;
       gj.key_bytes = kb;
       gj.mode = mode;
       gj.keys = data;
       gj.n = n;
-      gj.offsets = d_starts;
-      gj.ends = d_ends;
+      gj.offsets = d_se;
+      gj.ends = d_se + m;
       gj.n_segments = m;
+      gj.class_mask = mask;
       gj.counts = d_counts;
       gj.lists = d_lists;
       gj.num_cus = c->num_cus;
@@ -402,7 +405,7 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
       set_variants(c, kb, false, gj);
       if (e == hipSuccess) e = ts::segmented_sort(gj, s, &c->clk);
       if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
-      // The next round reuses d_starts / d_ends only after this sort (same stream).
+      // The next round reuses d_se only after this sort (same stream).
     }
     segs.swap(next);
   }

@@ -174,8 +174,13 @@ struct SegJob {
   // 0: classify and sort; 1: classify only (counts and lists filled);
   // 2: sort the classes of an earlier stage-1 call.
   int stage = 0;
+  // Size classes to sort (bit c: class c); a caller that knows some classes
+  // are empty may leave them out and save their launches.
+  uint32_t class_mask = ~0u;
 };
 uint32_t seg_lds_max_len(int key_bytes, bool vals);
+// Largest segment of size class c (0: a class this width does not use).
+uint32_t seg_class_cap(int key_bytes, bool vals, int size_class);
 // Kernel shapes a size class can run (variant 0 is the default), and a name
 // for each.
 int seg_variants(int key_bytes, bool vals, int size_class);

@@ -1096,15 +1096,15 @@ template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStrea
   }
   if (j.stage == 1) return hipGetLastError();
   // Longest segments first, so the short classes fill in behind them.
-  launch_class<U, VALS, 8>(a, j.num_cus, s, clk, j.variant[8]);
-  launch_class<U, VALS, 7>(a, j.num_cus, s, clk, j.variant[7]);
-  launch_class<U, VALS, 6>(a, j.num_cus, s, clk, j.variant[6]);
-  launch_class<U, VALS, 5>(a, j.num_cus, s, clk, j.variant[5]);
-  launch_class<U, VALS, 4>(a, j.num_cus, s, clk, j.variant[4]);
-  launch_class<U, VALS, 3>(a, j.num_cus, s, clk, j.variant[3]);
-  launch_class<U, VALS, 2>(a, j.num_cus, s, clk, j.variant[2]);
-  launch_class<U, VALS, 1>(a, j.num_cus, s, clk, j.variant[1]);
-  launch_class<U, VALS, 0>(a, j.num_cus, s, clk, j.variant[0]);
+  if (j.class_mask >> 8 & 1u) launch_class<U, VALS, 8>(a, j.num_cus, s, clk, j.variant[8]);
+  if (j.class_mask >> 7 & 1u) launch_class<U, VALS, 7>(a, j.num_cus, s, clk, j.variant[7]);
+  if (j.class_mask >> 6 & 1u) launch_class<U, VALS, 6>(a, j.num_cus, s, clk, j.variant[6]);
+  if (j.class_mask >> 5 & 1u) launch_class<U, VALS, 5>(a, j.num_cus, s, clk, j.variant[5]);
+  if (j.class_mask >> 4 & 1u) launch_class<U, VALS, 4>(a, j.num_cus, s, clk, j.variant[4]);
+  if (j.class_mask >> 3 & 1u) launch_class<U, VALS, 3>(a, j.num_cus, s, clk, j.variant[3]);
+  if (j.class_mask >> 2 & 1u) launch_class<U, VALS, 2>(a, j.num_cus, s, clk, j.variant[2]);
+  if (j.class_mask >> 1 & 1u) launch_class<U, VALS, 1>(a, j.num_cus, s, clk, j.variant[1]);
+  if (j.class_mask >> 0 & 1u) launch_class<U, VALS, 0>(a, j.num_cus, s, clk, j.variant[0]);
   return hipGetLastError();
 }
 
@@ -1115,6 +1115,17 @@ template <class U, bool VALS> uint32_t max_cap() {
 }
 
 }  // namespace
+
+uint32_t seg_class_cap(int key_bytes, bool vals, int size_class) {
+  if (size_class < 0 || size_class >= kSegDevice) return 0;
+  switch (key_bytes) {
+    case 1: return vals ? class_cap<uint8_t, true>(size_class) : class_cap<uint8_t, false>(size_class);
+    case 2: return vals ? class_cap<uint16_t, true>(size_class) : class_cap<uint16_t, false>(size_class);
+    case 4: return vals ? class_cap<uint32_t, true>(size_class) : class_cap<uint32_t, false>(size_class);
+    case 8: return vals ? class_cap<uint64_t, true>(size_class) : class_cap<uint64_t, false>(size_class);
+    default: return 0;
+  }
+}
 
 uint32_t seg_lds_max_len(int key_bytes, bool vals) {
   switch (key_bytes) {
