@@ -132,10 +132,10 @@ ts_status ts_segmented_set_variant(ts_ctx* ctx, ts_dtype dtype, int with_values,
  * be NULL (defaults); report may be NULL. */
 typedef struct ts_learned_cfg {
   uint32_t buckets_log2;     /* number of buckets = 2^buckets_log2 (1..24); default 16 */
-  uint32_t sample_per_mille; /* sample size as a fraction of n; default 10 (1 %) */
+  uint32_t sample_per_mille; /* sample size as a fraction of n; default 5 (0.5 %) */
   uint32_t train;            /* 1: fit the model (start from the sample's interval histogram, then Adam on the
                                 mean squared CDF error); 0: uniform knots (the ablation); default 1 */
-  uint32_t iters;            /* optimiser steps; default 100 */
+  uint32_t iters;            /* optimiser steps; default 40 */
   float lr;                  /* Adam step size; default 0.02 */
   uint32_t points;           /* training points taken from the sorted sample; default 16384 */
 } ts_learned_cfg;

@@ -62,7 +62,7 @@ def test_configurations(gpu, buckets_log2, train):
     cfg = ts.LearnedConfig(buckets_log2=buckets_log2, train=train, iters=50)
     rep = check_sorted(x, gpu, cfg)
     assert rep["buckets"] == 1 << buckets_log2
-    assert rep["sample"] == 4000
+    assert rep["sample"] == 400_000 * cfg.sample_per_mille // 1000
 
 
 def test_gradient_matches_oracle(gpu):

@@ -373,9 +373,9 @@ class LearnedConfig:
     """Settings of ``learned_sort`` (ts_learned_cfg)."""
 
     buckets_log2: int = 16
-    sample_per_mille: int = 10
+    sample_per_mille: int = 5
     train: bool = True
-    iters: int = 100
+    iters: int = 40
     lr: float = 0.02
     points: int = 16384
 

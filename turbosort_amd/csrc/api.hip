@@ -741,9 +741,9 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
 void ts_learned_cfg_default(ts_learned_cfg* cfg) {
   if (!cfg) return;
   cfg->buckets_log2 = 16;
-  cfg->sample_per_mille = 10;
+  cfg->sample_per_mille = 5;
   cfg->train = 1;
-  cfg->iters = 100;
+  cfg->iters = 40;
   cfg->lr = 0.02f;
   cfg->points = 16384;
 }
