@@ -556,14 +556,27 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
 // counting the run's smaller keys.  A segment with a run longer than
 // kMsdRunMax keys (a cluster of distinct keys: the count is quadratic in the
 // run) is finished by the byte-digit LSD passes of k_seg_radix8 instead, over
-// the bytes in which its keys differ.
-constexpr int kMsdBits = 12, kMsdBins = 1 << kMsdBits, kMsdRunMax = 64;
+// the bytes in which its keys differ.  The run counters are 16-bit halves of
+// LDS words (segments of at most 65535 keys), two bins per word.
+#ifndef TS_MSD_BITS
+#define TS_MSD_BITS 13
+#endif
+constexpr int kMsdBits = TS_MSD_BITS, kMsdBins = 1 << kMsdBits, kMsdRunMax = 64;
+
+__device__ __forceinline__ uint32_t msd_count(const uint32_t* cnt, uint32_t d) {
+  return (cnt[d >> 1] >> ((d & 1u) * 16u)) & 0xFFFFu;
+}
+// Adds one to bin d's counter and returns its old value.
+__device__ __forceinline__ uint32_t msd_bump(uint32_t* cnt, uint32_t d) {
+  const uint32_t sh = (d & 1u) * 16u;
+  return (atomicAdd(&cnt[d >> 1], 1u << sh) >> sh) & 0xFFFFu;
+}
 
 template <class U, int T, int IPT> struct SegMsdLds {
   static constexpr int CAP = T * IPT;
   alignas(16) U keys[CAP];
   union {
-    uint32_t cnt[kMsdBins];     // run counts, then starts, then ends
+    uint32_t cnt[kMsdBins / 2];  // run counts, then starts, then ends (16-bit halves)
     uint32_t whist[T / 64][256];  // the LSD passes' per-wave digit counts
   };
   uint32_t wsum[T / 64];
@@ -587,7 +600,8 @@ template <int T> __device__ __forceinline__ uint32_t block_exclusive_add(uint32_
 
 template <class U, int T, int IPT>
 __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
-  static_assert(T >= 256 && T % 64 == 0 && kMsdBins % T == 0, "lanes must divide the bins");
+  static_assert(T >= 256 && T % 64 == 0 && kMsdBins % (2 * T) == 0, "lanes must divide the bins");
+  static_assert(T * IPT < 65536, "16-bit run counters");
   constexpr int NW = T / 64, PER = kMsdBins / T;
   constexpr int P = int(sizeof(U));
   constexpr int CAP = T * IPT;
@@ -614,7 +628,7 @@ __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
         mn = k[j] < mn ? k[j] : mn;
         mx_k = k[j] > mx_k ? k[j] : mx_k;
       }
-    for (unsigned i = tid; i < unsigned(kMsdBins); i += T) L.cnt[i] = 0;
+    for (unsigned i = tid; i < unsigned(kMsdBins / 2); i += T) L.cnt[i] = 0;
     if (tid == 0) {
       L.flags[0] = 0;
       L.flags[1] = 0;
@@ -654,13 +668,18 @@ __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
     auto dig = [&](U x) { return uint32_t(uint64_t(U(x - kmin)) >> sh); };
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
-      if (uint32_t(j) * T + tid < len) atomicAdd(&L.cnt[dig(k[j])], 1u);
+      if (uint32_t(j) * T + tid < len) msd_bump(L.cnt, dig(k[j]));
     __syncthreads();
     // Run starts (digits in order) and the longest run.
     uint32_t c[PER], sum = 0, mx = 0;
 #pragma unroll
+    for (int q = 0; q < PER / 2; ++q) {
+      const uint32_t wv = L.cnt[tid * (PER / 2) + q];
+      c[2 * q] = wv & 0xFFFFu;
+      c[2 * q + 1] = wv >> 16;
+    }
+#pragma unroll
     for (int q = 0; q < PER; ++q) {
-      c[q] = L.cnt[tid * PER + q];
       sum += c[q];
       mx = max(mx, c[q]);
     }
@@ -669,16 +688,18 @@ __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
     if (lane == 0) atomicMax(&L.flags[2], mx);
     uint32_t run = block_exclusive_add<T>(sum, L.wsum);
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      L.cnt[tid * PER + q] = run;
-      run += c[q];
+    for (int q = 0; q < PER / 2; ++q) {
+      const uint32_t lo = run;
+      run += c[2 * q];
+      L.cnt[tid * (PER / 2) + q] = lo | (run << 16);
+      run += c[2 * q + 1];
     }
     __syncthreads();
     // Scatter: a key's slot is its digit's next free one (any order inside a run).
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
-      if (uint32_t(j) * T + tid < len) L.keys[atomicAdd(&L.cnt[dig(k[j])], 1u)] = k[j];
-    __syncthreads();  // L.cnt[d] is now the end of run d
+      if (uint32_t(j) * T + tid < len) L.keys[msd_bump(L.cnt, dig(k[j]))] = k[j];
+    __syncthreads();  // bin d's counter is now the end of run d
     if (L.flags[2] <= uint32_t(kMsdRunMax)) {
       // Each key's final slot: its run's start plus the keys of the run that
       // are smaller, or equal and earlier (independent reads, no chains);
@@ -691,7 +712,7 @@ __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
         if (p < len) {
           const U x = L.keys[p];
           const uint32_t d = dig(x);
-          const uint32_t b = d ? L.cnt[d - 1] : 0u, e = L.cnt[d];
+          const uint32_t b = d ? msd_count(L.cnt, d - 1) : 0u, e = msd_count(L.cnt, d);
           uint32_t r = b;
           for (uint32_t q = b; q < e; ++q) {
             const U y = L.keys[q];
