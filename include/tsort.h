@@ -211,6 +211,14 @@ enum { TS_RANK_BALLOT = 0, TS_RANK_ATOMIC = 1 };
 ts_status ts_ctx_set_ranking(ts_ctx* ctx, int method);
 int ts_ctx_ranking(const ts_ctx* ctx);
 
+/* Keys-only segments longer than one workgroup holds (segmented sort without
+ * values, the learned sort's overflowing buckets) go through rounds of
+ * three-way splits; what is still too long after this many rounds (default
+ * 12, 0..64) takes the device radix sort.  No counterpart in the reference.
+ * Any value gives the same output. */
+ts_status ts_ctx_set_split_rounds(ts_ctx* ctx, int rounds);
+int ts_ctx_split_rounds(const ts_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,24 @@ def test_trained_model_balances_buckets_better_than_uniform_knots(gpu):
     assert plain["largest_bucket"] > 2 * trained["largest_bucket"], (plain, trained)
 
 
+@pytest.mark.parametrize("rounds", [0, 1])
+def test_overflow_without_split_rounds(gpu, rounds):
+    """Overflowing buckets that the split rounds leave take the device radix
+    sort (on the learned sort's second stream); the output stays exact."""
+    n = 2_000_000
+    g = rng(81)
+    x = random_of("u64", n, seed=82)
+    hot = random_of("u64", 2, seed=83)
+    x = np.where(g.random(n) < 0.4, hot[0], np.where(g.random(n) < 0.2, hot[1], x)).astype(np.uint64)
+    ctx = ts.Context(gpu)
+    ctx.set_split_rounds(rounds)
+    t = to_dev(x, gpu)
+    rep = ts.learned_sort(t, ctx=ctx)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(t, np.uint64), np.sort(x))
+    assert rep["overflow_buckets"] >= 1 and rep["split_rounds"] <= rounds, rep
+
+
 def test_rejects_other_widths(gpu):
     t = torch.zeros(10, dtype=torch.uint16, device=gpu)
     with pytest.raises(ValueError, match="4- and 8-byte"):

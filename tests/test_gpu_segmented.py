@@ -176,6 +176,28 @@ def test_very_long_keys_only_segments(gpu, dtype):
     check(keys, off, None, gpu)
 
 
+@pytest.mark.parametrize("rounds", [0, 1, 2])
+def test_split_round_limit_hands_the_rest_to_the_radix_sort(gpu, rounds):
+    """Long keys-only segments left after the set number of split rounds take
+    the device radix sort; the output does not depend on the limit."""
+    cap = ts.segmented_lds_max_len(torch.uint32, False)
+    g = rng(71)
+    a = random_of("u32", 40 * cap, seed=72)
+    b = random_of("u32", 9, seed=73)[g.integers(0, 9, size=5 * cap)]
+    keys = np.concatenate([a, b, random_of("u32", 2 * cap + 3, seed=74), random_of("u32", 500, seed=75)])
+    off = offsets_of([40 * cap, 5 * cap, 2 * cap + 3, 500])
+    ctx = ts.Context(gpu)
+    ctx.set_split_rounds(rounds)
+    assert ctx.split_rounds == rounds
+    tk, to = to_dev(keys, gpu), to_dev(off, gpu)
+    ts.segmented_sort(tk, to, None, ctx=ctx)
+    torch.cuda.synchronize()
+    want, _ = cf.segmented_sort(keys, off, None)
+    assert np.array_equal(to_host(tk, np.uint32), want)
+    with pytest.raises(ValueError):
+        ctx.set_split_rounds(65)
+
+
 @pytest.mark.parametrize("with_values", [False, True])
 def test_more_long_segments_than_one_readback(gpu, with_values):
     """More segments past the LDS capacity than the first bounds readback

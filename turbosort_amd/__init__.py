@@ -116,6 +116,15 @@ class Context:
     def set_ranking(self, method: str) -> None:
         self.check(self._lib.ts_ctx_set_ranking(self._h, self.RANKINGS.index(method)))
 
+    @property
+    def split_rounds(self) -> int:
+        """Three-way split rounds that long keys-only segments (and the
+        learned sort's overflowing buckets) take before the device radix sort."""
+        return int(self._lib.ts_ctx_split_rounds(self._h))
+
+    def set_split_rounds(self, rounds: int) -> None:
+        self.check(self._lib.ts_ctx_set_split_rounds(self._h, int(rounds)))
+
     def kernel_times(self):
         """[(kernel name, ms, algorithmic bytes)] of the last call, from HIP events."""
         cap = 256

@@ -20,7 +20,7 @@ EXPORTS = [
     "ts_sbk_last_error", "ts_segmented_sort_pairs", "ts_segmented_lds_max_len",
     "ts_learned_cfg_default", "ts_learned_sort", "ts_pwl_fit",
     "ts_segmented_variants", "ts_segmented_variant_name", "ts_segmented_set_variant",
-    "ts_ctx_set_ranking", "ts_ctx_ranking", "ts_sort_host",
+    "ts_ctx_set_ranking", "ts_ctx_ranking", "ts_sort_host", "ts_ctx_set_split_rounds", "ts_ctx_split_rounds",
 ]
 
 
@@ -108,6 +108,10 @@ def load() -> ctypes.CDLL:
     L.ts_ctx_set_ranking.restype = i32
     L.ts_ctx_ranking.argtypes = [vp]
     L.ts_ctx_ranking.restype = i32
+    L.ts_ctx_set_split_rounds.argtypes = [vp, i32]
+    L.ts_ctx_set_split_rounds.restype = i32
+    L.ts_ctx_split_rounds.argtypes = [vp]
+    L.ts_ctx_split_rounds.restype = i32
     L.ts_ctx_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_double), i32]
     L.ts_ctx_kernel_times.restype = i32

@@ -31,6 +31,8 @@ struct ts_ctx {
   // lane order (rank_order_check), and whether the sorts use that ranking.
   bool atomic_rank_ok = false;
   bool atomic_rank = false;
+  // Three-way split rounds for long keys-only segments before the radix sort.
+  int split_rounds = 12;
   // ts_sort_host: device copy of the caller's host slice (grow-only).
   char* hdev = nullptr;
   size_t hdev_bytes = 0;
@@ -290,7 +292,7 @@ size_t long_radix_bytes(size_t n, int kb) { return up(sizeof(ts::SortMeta)) + up
 ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, size_t n,
                          std::vector<std::pair<uint64_t, uint64_t>> segs, char* area, char* radix_ws, char* pin,
                          size_t f, hipStream_t s, uint32_t* rounds) {
-  constexpr int kRounds = 12;
+  const uint32_t kRounds = uint32_t(c->split_rounds);
   const uint32_t cap = ts::seg_lds_max_len(kb, false);
   auto radix_part = [&](uint64_t lo, uint64_t len) {
     ts::RadixJob j;
@@ -335,7 +337,7 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
   uint32_t class_cap[ts::kSegDevice];
   for (int k = 0; k < ts::kSegDevice; ++k) class_cap[k] = ts::seg_class_cap(kb, false, k);
   uint32_t r = 0;
-  for (; r < uint32_t(kRounds) && !segs.empty(); ++r) {
+  for (; r < kRounds && !segs.empty(); ++r) {
     const size_t ns = segs.size();
     if (ns > f) return fail(c, TS_ERR_INVALID_ARG, "internal: too many long segments");
     auto* h_lo = reinterpret_cast<unsigned long long*>(h_seg);
@@ -495,6 +497,15 @@ ts_status ts_ctx_set_ranking(ts_ctx* c, int method) {
 }
 
 int ts_ctx_ranking(const ts_ctx* c) { return c && c->atomic_rank ? TS_RANK_ATOMIC : TS_RANK_BALLOT; }
+
+ts_status ts_ctx_set_split_rounds(ts_ctx* c, int rounds) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (rounds < 0 || rounds > 64) return fail(c, TS_ERR_INVALID_ARG, "split rounds: 0 to 64");
+  c->split_rounds = rounds;
+  return TS_OK;
+}
+
+int ts_ctx_split_rounds(const ts_ctx* c) { return c ? c->split_rounds : -1; }
 
 void ts_ctx_destroy(ts_ctx* c) {
   if (!c) return;
