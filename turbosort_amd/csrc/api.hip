@@ -283,8 +283,9 @@ size_t long_radix_bytes(size_t n, int kb) { return up(sizeof(ts::SortMeta)) + up
 // so no order among them is observable): rounds of three-way splits around a
 // pivot; the parts that fit one workgroup go to the segmented LDS classes,
 // longer parts are split again (all of a round's segments in the same
-// launches).  What is left after kRounds rounds takes the device radix sort,
-// with the segment's own stretch of `tmp` as its second buffer.  `tmp` holds n keys; `area` has long_keys_bytes(f) bytes,
+// launches).  What is left after the context's split_rounds rounds takes the
+// device radix sort, with the segment's own stretch of `tmp` as its second
+// buffer.  `tmp` holds n keys; `area` has long_keys_bytes(f) bytes,
 // `radix_ws` long_radix_bytes(n, kb) bytes and `pin` long_keys_pin_bytes(f)
 // bytes of pinned host memory, with f >= long_keys_max(n, cap).  Nothing else
 // of the context's workspace is touched, so the caller may still be using
