@@ -59,7 +59,12 @@ __global__ void k_init(SortMeta* m, int P) {
 // agree -- constant digits, sorted runs -- is counted with one LDS add
 // instead of 64 colliding atomics.  The grid also clears the first look-back
 // row.
-constexpr int kHistThreads = 256;
+#ifndef TS_HIST_THREADS
+#define TS_HIST_THREADS 256
+#endif
+#ifndef TS_HIST_THREADS_WIDE
+#define TS_HIST_THREADS_WIDE 1024
+#endif
 #ifndef TS_HIST_UNROLL_NARROW
 #define TS_HIST_UNROLL_NARROW 16
 #endif
@@ -70,16 +75,18 @@ constexpr int kHistThreads = 256;
 #define TS_HIST_BLOCKS_PER_CU 4
 #endif
 #ifndef TS_HIST_BLOCKS_PER_CU_WIDE
-#define TS_HIST_BLOCKS_PER_CU_WIDE 2
+#define TS_HIST_BLOCKS_PER_CU_WIDE 1
 #endif
 #ifndef TS_HIST_BLOCKS_PER_CU_LEARNED
 #define TS_HIST_BLOCKS_PER_CU_LEARNED 4
 #endif
 // Measured on MI355X at 2^28 keys: 16 keys per lane per step; 8-byte keys
-// (eight digit tables per workgroup) prefer fewer, longer workgroups.
+// (eight digit tables per workgroup) prefer one 1024-lane workgroup per CU
+// (0.55 ms against 0.65 ms for two of 256 lanes), 4-byte keys four of 256.
 template <class U> struct HistUnroll {
   static constexpr int value = sizeof(U) >= 8 ? TS_HIST_UNROLL_WIDE : TS_HIST_UNROLL_NARROW;
   static constexpr int blocks_per_cu = sizeof(U) >= 8 ? TS_HIST_BLOCKS_PER_CU_WIDE : TS_HIST_BLOCKS_PER_CU;
+  static constexpr int threads = sizeof(U) >= 8 ? TS_HIST_THREADS_WIDE : TS_HIST_THREADS;
 };
 
 // The digits of a pass come from the radix key itself, or (LEARNED) from the
@@ -157,7 +164,7 @@ __global__ __launch_bounds__(256) void k_order_check(const U* __restrict__ keys,
 }
 
 template <class U, bool REC, bool LEARNED = false>
-__global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ keys, const unsigned char* __restrict__ rec,
+__global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __restrict__ keys, const unsigned char* __restrict__ rec,
                                                        size_t stride, size_t key_off, U* __restrict__ keys_out,
                                                        size_t n, int mode, size_t seg0, uint32_t blocks_per_seg,
                                                        size_t per_block, SortMeta* m, uint32_t* zero_ptr,
@@ -166,6 +173,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const U* __restrict__ key
   using D = typename DigitSource<U, LEARNED>::T;
   constexpr int P = DigitSource<U, LEARNED>::P;
   constexpr int UN = HistUnroll<U>::value;
+  constexpr int kHistThreads = HistUnroll<U>::threads;
   // Keys already in order (k_order_check; check_order as k_scan's presorted):
   // no pass runs, so no counts.
   if (!LEARNED && check_order && (!m->breaks_up || (check_order == 2 && !m->breaks_down))) return;
@@ -797,9 +805,12 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   // contiguous chunk of it.
   const bool vals = j.final_vals != nullptr;
   const size_t seg0 = first_segment(j.n, KB, LEARNED, vals);
-  // The learned histogram holds three digit tables (and the knots), not
-  // eight: more of its workgroups fit a CU.
-  const int hist_per_cu = LEARNED ? TS_HIST_BLOCKS_PER_CU_LEARNED : HistUnroll<U>::blocks_per_cu;
+  // Histogram workgroups per CU.  The learned histogram holds three digit
+  // tables (and the knots), not eight, so more fit a CU;
+  // TS_HIST_BLOCKS_PER_CU_LEARNED counts 256-lane workgroups.
+  constexpr int kHistThreads = HistUnroll<U>::threads;
+  const int hist_per_cu = LEARNED ? std::max(1, TS_HIST_BLOCKS_PER_CU_LEARNED * 256 / kHistThreads)
+                                  : HistUnroll<U>::blocks_per_cu;
   const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * hist_per_cu / kSegments));
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
   size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
