@@ -482,6 +482,11 @@ ts_status ts_ctx_create(ts_ctx** out, int device) {
   // the MSD shape by default (measured on MI355X: 27.4 vs 28.1 ms for the
   // learned sort of 10^9 u64 keys).
   for (int w = 2; w < 4; ++w) c->seg_variant[w][0][8] = 2;
+  // 4-byte keys with values, segments of 257-1024 and 1025-2048 keys: byte
+  // digits (the shapes the autotuner picks on MI355X for the log-uniform
+  // batch of 10^5 segments).
+  c->seg_variant[2][1][4] = 1;
+  c->seg_variant[2][1][5] = 1;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     c->num_cus = cus;
