@@ -478,10 +478,9 @@ ts_status ts_ctx_create(ts_ctx** out, int device) {
   c->device = device;
   c->atomic_rank_ok = ok;
   c->atomic_rank = ok;
-  // Keys-only segments of the largest class (learned buckets among them):
-  // the MSD shape by default (measured on MI355X: 27.4 vs 28.1 ms for the
-  // learned sort of 10^9 u64 keys).
-  for (int w = 2; w < 4; ++w) c->seg_variant[w][0][8] = 2;
+  // Keys-only 4-byte segments of the largest class: the MSD shape by
+  // default (8-byte keys have no other shape for that class).
+  c->seg_variant[2][0][8] = 2;
   // 4-byte keys with values, segments of 257-1024 and 1025-2048 keys: byte
   // digits (the shapes the autotuner picks on MI355X for the log-uniform
   // batch of 10^5 segments).

@@ -995,6 +995,9 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
                                : ClassShape{};
     case 8:
       if (wide && VALS) return ClassShape{};
+      // 8-byte keys without values: the MSD shape only, with 17 keys per
+      // lane (17408 keys in 136 KiB), which the other shapes' LDS cannot hold.
+      if (wide) return v == 0 ? ClassShape{0, 1024, 17, true, true} : ClassShape{};
       return v == 0   ? ClassShape{0, 1024, 16, true}
              : !alt   ? ClassShape{}
              : v == 1 ? ClassShape{0, 512, 32, false}
