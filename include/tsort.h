@@ -113,11 +113,15 @@ ts_status ts_segmented_sort_pairs(ts_ctx* ctx, ts_dtype dtype, void* d_keys, uin
 size_t ts_segmented_lds_max_len(ts_dtype dtype, int with_values);
 
 /* Autotuning hooks.  Size class c (0 .. 8: segments of up to 16, 32, 64, 256,
- * 1024, 2048, 4096, 8192, 16384 keys) can run one of
- * ts_segmented_variants(...) kernel shapes, named by
- * ts_segmented_variant_name(...); variant 0 is the default.  The choice is
- * kept per context and per key width (with / without values); the learned
- * sort's bucket sorts use the keys-only choice. */
+ * 1024, 2048, 4096, 8192 and 16384 keys; the last class holds 17408 8-byte
+ * keys without values and does not exist for 8-byte keys with values) can
+ * run one of ts_segmented_variants(...) kernel shapes, named by
+ * ts_segmented_variant_name(...).  A new context starts from the shapes
+ * measured fastest on MI355X (variant 0 except: byte digits for 4-byte keys
+ * with values in classes 4 and 5, the MSD shape for 4-byte keys without
+ * values in class 8).  The choice is kept per context and per key width
+ * (with / without values); the learned sort's bucket sorts use the
+ * keys-only choice. */
 int ts_segmented_variants(ts_dtype dtype, int with_values, int size_class);
 const char* ts_segmented_variant_name(ts_dtype dtype, int with_values, int size_class, int variant);
 ts_status ts_segmented_set_variant(ts_ctx* ctx, ts_dtype dtype, int with_values, int size_class, int variant);
