@@ -84,3 +84,20 @@ def test_segmented_variant_tables_are_consistent():
             cap = L.ts_segmented_lds_max_len(code, vals)
             assert cap == ((8192 if vals else 17408) if width == 8 else 16384)
     assert L.ts_segmented_variants(2, 1, 9) == 0 and L.ts_segmented_variants(2, 1, -1) == 0
+
+
+def test_learned_defaults_agree_across_layers():
+    """ts_learned_cfg_default (host-only) and the Python LearnedConfig give
+    the same defaults."""
+    import ctypes
+
+    import turbosort_amd as ts
+    from turbosort_amd import _native
+
+    L = _native.load()
+    c = _native.LearnedCfg()
+    L.ts_learned_cfg_default(ctypes.byref(c))
+    py = ts.LearnedConfig()
+    assert (c.buckets_log2, c.sample_per_mille, bool(c.train), c.iters, c.points) == (
+        py.buckets_log2, py.sample_per_mille, py.train, py.iters, py.points)
+    assert abs(c.lr - py.lr) < 1e-7
