@@ -397,9 +397,9 @@ constexpr uint32_t kSpinLimit = 1u << 26;
 // pass applies the inverse key transform, and in a by-key sort writes only the
 // order to final_vals.
 // LDS of one scatter tile, declared once in the kernel.
-template <class U, bool VALS, int IPT, bool LEARNED = false> struct SweepLds {
-  static constexpr int NW = SweepShape<sizeof(U), LEARNED, VALS>::threads / 64;
-  static constexpr int TILE = SweepShape<sizeof(U), LEARNED, VALS>::threads * IPT;
+template <class U, bool VALS, int IPT, bool LEARNED = false, bool SMALL = false> struct SweepLds {
+  static constexpr int NW = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads / 64;
+  static constexpr int TILE = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads * IPT;
   uint32_t whist[NW][256];  // per-wave digit counts, then staging bases
   U keys[TILE];
   uint32_t vals[VALS ? TILE : 1];
@@ -414,14 +414,14 @@ template <class U, bool VALS, int IPT, bool LEARNED = false> struct SweepLds {
 // Everything a tile does after it knows where it is: load, publish its digit
 // counts, rank, look back, stage, store.  FULL tiles (all but each segment's
 // last) run without any per-key bounds test.
-template <class U, bool VALS, class S, bool FULL, bool LEARNED>
-__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS>::ipt, LEARNED>& L,
+template <class U, bool VALS, class S, bool FULL, bool LEARNED, bool SMALL>
+__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS, SMALL>::ipt, LEARNED, SMALL>& L,
                                            const SweepArgs<U>& a,
                                            S* __restrict__ status, int pass, unsigned shift, bool first, bool last,
                                            size_t tile_base, uint32_t valid, size_t first_slot, size_t slot,
                                            bool first_round, unsigned long long digit_base) {
-  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS>::ipt;
-  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS>::threads;
+  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
   constexpr int VB = Flags<S>::kBits;
@@ -628,14 +628,14 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
 // Pass 0 of a by-key sort generates the values (positions 0..n-1); the last
 // pass applies the inverse key transform, and in a by-key sort writes only the
 // order to final_vals.
-template <class U, bool VALS, class S, bool LEARNED = false>
-__global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS>::threads)) void k_sweep(SweepArgs<U> a,
-                                                                                  S* __restrict__ status, int pass) {
-  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS>::ipt;
-  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS>::threads;
+template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false>
+__global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads)) void k_sweep(
+    SweepArgs<U> a, S* __restrict__ status, int pass) {
+  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
-  __shared__ SweepLds<U, VALS, IPT, LEARNED> L;
+  __shared__ SweepLds<U, VALS, IPT, LEARNED, SMALL> L;
 
   const SortMeta* __restrict__ meta = a.meta;
   const uint32_t n_live = meta->n_live;
@@ -709,11 +709,11 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS>::threads)) vo
   const unsigned long long digit_base =
       tid < 256 ? meta->base[digit][tid] + meta->segpre[digit][seg][tid] : 0ull;
   if (valid == uint32_t(TILE))
-    run_tile<U, VALS, S, true, LEARNED>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
-                                          slot, round == 0, digit_base);
+    run_tile<U, VALS, S, true, LEARNED, SMALL>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
+                                                 slot, round == 0, digit_base);
   else
-    run_tile<U, VALS, S, false, LEARNED>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
-                                           slot, round == 0, digit_base);
+    run_tile<U, VALS, S, false, LEARNED, SMALL>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
+                                                  slot, round == 0, digit_base);
 }
 
 // Non-increasing keys found by the scan: reversing them sorts them.  Each
@@ -790,6 +790,9 @@ template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& 
     else if (vals)
       hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U), false, true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
+    else if (small_sweep(j.n, false, false))
+      hipLaunchKernelGGL((k_sweep<U, false, S, false, true>), dim3(tiles),
+                         dim3(SweepShape<sizeof(U), false, false, true>::threads), 0, s, a, static_cast<S*>(j.status), p);
     else
       hipLaunchKernelGGL((k_sweep<U, false, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
     if (clk) clk->end(s);
@@ -846,7 +849,8 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
                        size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
                        static_cast<uint32_t*>(j.status), zero_words, nullptr, check_order);
   if (clk) clk->end(s);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, uint32_t(sweep_tile(KB, LEARNED, vals)),
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0,
+                     uint32_t(sweep_tile(KB, LEARNED, vals, small_sweep(j.n, LEARNED, vals))),
                      check_order);
   hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk)
                                   : run_sweeps<U, uint32_t, LEARNED>(j, s, clk);
