@@ -2,7 +2,8 @@
 // turbosort::sort (lib.rs:99-101) -- the unique ascending order of the radix
 // keys -- reached by bucketing instead of by eight byte passes.
 //
-//   1. A stratified 1 % sample of the radix keys, sorted by the device radix.
+//   1. A stratified sample of the radix keys (0.5 % by default), sorted by
+//      the device radix.
 //   2. A monotone piecewise-linear CDF (kKnots knots over a logarithmic key
 //      coordinate) fitted to the sample's ranks by gradient descent (Adam) in
 //      one workgroup: the knot heights are normalised cumulative softplus of
@@ -36,9 +37,15 @@ template <class U>
 __global__ void k_learn_sample(const U* __restrict__ data, size_t n, int mode, unsigned long long* __restrict__ out,
                                size_t S) {
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  // Stratum j is [j n / S, (j + 1) n / S) in double precision (no 64-bit
+  // division, no overflow of j * n); the pick inside it is the high word of
+  // a hash times the stratum's length, and is clamped into the keys.
+  const double step = double(n) / double(S);
   for (size_t j = g; j < S; j += gs) {
-    const size_t lo = j * n / S, hi = (j + 1) * n / S;
-    const size_t i = lo + size_t(mix64(0x5EEDull + j) % (unsigned long long)(hi - lo));
+    const size_t lo = min(size_t(double(j) * step), n - 1);
+    const size_t hi = min(size_t(double(j + 1) * step), n);
+    const unsigned long long span = hi > lo ? (unsigned long long)(hi - lo) : 1ull;
+    const size_t i = min(n - 1, lo + size_t(__umul64hi(mix64(0x5EEDull + j), span)));
     out[j] = (unsigned long long)to_key<U>(data[i], mode);
   }
 }
