@@ -71,6 +71,15 @@ def test_random_every_dtype_and_tier(gpu, name):
 
 
 @pytest.mark.parametrize("name", cf.DTYPES)
+def test_both_tile_shapes_every_dtype(gpu, name):
+    """Keys-only sorts up to 2^21 keys run small scatter tiles, larger ones
+    the large tiles: both sides of the switch and a ragged larger size."""
+    for n in (1 << 21, (1 << 21) + 1, 3_000_017):
+        check(random_of(name, n, seed=n + 7), gpu, f"{name} n={n}")
+    check(random_of(name, 3_000_017, seed=11), gpu, f"{name} buffer", buffer_len=3_000_017)
+
+
+@pytest.mark.parametrize("name", cf.DTYPES)
 def test_sort_with_buffer_every_dtype(gpu, name):
     for n in (17, 513, 4097, 70_000):
         check(random_of(name, n, seed=n), gpu, f"{name} n={n}", buffer_len=n + 5)
