@@ -216,7 +216,7 @@ def test_sort_host(gpu, pinned):
         ts.sort_host(torch.zeros(4, dtype=torch.uint32, device=gpu), ctx=ctx)
 
 
-@pytest.mark.parametrize("n", [4097, 100_000, 1_000_003])
+@pytest.mark.parametrize("n", [4097, 100_000, 1_000_003, 2_500_001])
 def test_presorted_shortcuts(gpu, n):
     """Non-decreasing input skips every pass and non-increasing input is
     reversed (presorted.rs:15-65); any single out-of-place pair, including at
