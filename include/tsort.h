@@ -114,7 +114,7 @@ size_t ts_segmented_lds_max_len(ts_dtype dtype, int with_values);
 
 /* Autotuning hooks.  Size class c (0 .. 8: segments of up to 16, 32, 64, 256,
  * 1024, 2048, 4096, 8192 and 16384 keys; the last class holds 17408 8-byte
- * keys without values and does not exist for 8-byte keys with values) can
+ * keys without values and 11264 with values) can
  * run one of ts_segmented_variants(...) kernel shapes, named by
  * ts_segmented_variant_name(...).  A new context starts from the shapes
  * measured fastest on MI355X (variant 0 except: byte digits for 4-byte keys

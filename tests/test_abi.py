@@ -64,7 +64,7 @@ def test_library_has_no_oracle_dependency():
 def test_segmented_variant_tables_are_consistent():
     """Host-only queries (no GPU call): every size class has a default shape,
     4- and 8-byte keys offer alternatives, 1- and 2-byte keys do not, and
-    8-byte keys stop at 8192 keys with values and at 17408 without."""
+    8-byte keys stop at 11264 keys with values and at 17408 without."""
     from turbosort_amd import _native
 
     L = _native.load()
@@ -82,7 +82,7 @@ def test_segmented_variant_tables_are_consistent():
                 for v in range(n):
                     assert L.ts_segmented_variant_name(code, vals, c, v)
             cap = L.ts_segmented_lds_max_len(code, vals)
-            assert cap == ((8192 if vals else 17408) if width == 8 else 16384)
+            assert cap == ((11264 if vals else 17408) if width == 8 else 16384)
     assert L.ts_segmented_variants(2, 1, 9) == 0 and L.ts_segmented_variants(2, 1, -1) == 0
 
 

@@ -23,7 +23,8 @@ pytestmark = pytest.mark.gpu
 # Both sides of every class bound (16, 32, 64, 256, 1024, ..., 16384) and the
 # device class beyond the LDS capacity.
 EDGE_LENGTHS = [0, 1, 2, 3, 5, 15, 16, 17, 31, 32, 33, 63, 64, 65, 200, 255, 256, 257, 1000, 1024, 1025, 2047,
-                2048, 2049, 4095, 4096, 4097, 8191, 8192, 8193, 12000, 16383, 16384, 16385, 17408, 17409, 40000]
+                2048, 2049, 4095, 4096, 4097, 8191, 8192, 8193, 11264, 11265, 12000, 16383, 16384, 16385, 17408, 17409,
+                40000]
 
 
 def offsets_of(lengths, lead: int = 0) -> np.ndarray:

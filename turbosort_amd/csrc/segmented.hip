@@ -994,7 +994,9 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
              : v == 3 && !VALS ? ClassShape{0, 512, 16, true, true}
                                : ClassShape{};
     case 8:
-      if (wide && VALS) return ClassShape{};
+      // 8-byte keys with values: 11264 pairs (132 KiB) are what one
+      // workgroup's LDS holds.
+      if (wide && VALS) return v == 0 ? ClassShape{0, 1024, 11, true} : ClassShape{};
       // 8-byte keys without values: the MSD shape only, with 17 keys per
       // lane (17408 keys in 136 KiB), which the other shapes' LDS cannot hold.
       if (wide) return v == 0 ? ClassShape{0, 1024, 17, true, true} : ClassShape{};
