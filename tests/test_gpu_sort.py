@@ -235,3 +235,34 @@ def test_presorted_shortcuts(gpu, n):
             z = x[::-1].copy()
             z[pos - 1], z[pos] = z[pos], z[pos - 1]
             check(z, gpu, f"{name} descending, pair {pos} swapped")
+
+
+@pytest.mark.parametrize("name,n", [("u32", 1_000_003), ("u32", 3_000_017), ("f64", 200_000), ("i16", 70_000),
+                                    ("u64", 3_000)])
+def test_sort_replays_from_a_graph(gpu, name, n):
+    """sort makes no host call once the workspace has grown, so it can be
+    captured in a graph; every replay sorts freshly restored input exactly,
+    with eager work and synchronisations between the replays."""
+    x = random_of(name, n, seed=n + 3)
+    want = bits(cf.sort(x))
+    t = to_dev(x, gpu)
+    src = t.clone()
+    ctx = ts.Context(gpu)
+    ts.sort(t, ctx=ctx)  # grows the workspace before capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        t.copy_(src)
+        with torch.cuda.graph(g, stream=s):
+            ts.sort(t, ctx=ctx)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        t.copy_(src)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(to_host(t, x.dtype)), want)
+
