@@ -38,8 +38,7 @@ struct KernelClock {
 // multiple of 256; the first 256 lanes also carry one digit each through the
 // per-digit steps) and `ipt` keys per lane, per key width (4-byte and
 // narrower / 8-byte) and kind: keys only, keys with values (both staged in
-// LDS), keys ordered by learned buckets (whose staged digits and model knots
-// also take LDS).  Measured on MI355X at 2^28 keys with LDS-atomic ranking:
+// LDS), keys ordered by learned buckets (whose model knots also take LDS).  Measured on MI355X at 2^28 keys with LDS-atomic ranking:
 // the passes are bound by memory latency and the scattered write runs, and
 // larger tiles (longer runs per digit, fewer look-back steps) win down to one
 // workgroup per CU.
@@ -59,7 +58,7 @@ struct KernelClock {
 #define TS_SWEEP_LEARNED_NARROW 512, 32
 #endif
 #ifndef TS_SWEEP_LEARNED_WIDE
-#define TS_SWEEP_LEARNED_WIDE 1024, 14
+#define TS_SWEEP_LEARNED_WIDE 1024, 16
 #endif
 // Keys-only sorts of at most TS_SWEEP_SMALL_N keys use 8192-key tiles, so
 // that their tiles cover more CUs (measured on MI355X, u32: 100 against

@@ -403,7 +403,6 @@ template <class U, bool VALS, int IPT, bool LEARNED = false, bool SMALL = false>
   uint32_t whist[NW][256];  // per-wave digit counts, then staging bases
   U keys[TILE];
   uint32_t vals[VALS ? TILE : 1];
-  uint8_t sdig[LEARNED ? TILE : 1];  // staged keys' digits (learned digits are not a key field)
   unsigned long long knots[LEARNED ? kKnots : 1];
   unsigned long long gofs[256];
   uint32_t wave[NW];
@@ -529,7 +528,6 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
       const uint32_t pos = L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
       L.keys[pos] = k[j];
       if (VALS) L.vals[pos] = v[j];
-      if (LEARNED) L.sdig[pos] = uint8_t(d);
     }
     // Keep the scheduler from hoisting every iteration's LDS read ahead of
     // the writes, which costs more registers than it hides latency.
@@ -591,7 +589,10 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
       const uint32_t s = uint32_t(j) * THREADS + tid;
       if (FULL || s < valid) {
         const U kk = L.keys[s];
-        const uint32_t d = LEARNED ? uint32_t(L.sdig[s]) : digit_of(kk, shift);
+        // Learned digits are evaluated again rather than staged: the pass
+        // is far from VALU-bound, and the LDS goes to a larger tile.
+        const uint32_t d = LEARNED ? digit_of(bucket_of(kk, L.knots, *a.model, a.model->bucket_shift), shift)
+                                   : digit_of(kk, shift);
         const size_t o = size_t(L.gofs[d]) + s;
         if (keys_out) dst[o] = transform(kk);
         if (VALS) vdst[o] = L.vals[s];
