@@ -49,10 +49,10 @@ struct KernelClock {
 #define TS_SWEEP_KEYS_WIDE 512, 36
 #endif
 #ifndef TS_SWEEP_VALS_NARROW
-#define TS_SWEEP_VALS_NARROW 512, 32
+#define TS_SWEEP_VALS_NARROW 512, 36
 #endif
 #ifndef TS_SWEEP_VALS_WIDE
-#define TS_SWEEP_VALS_WIDE 512, 16
+#define TS_SWEEP_VALS_WIDE 512, 24
 #endif
 #ifndef TS_SWEEP_LEARNED_NARROW
 #define TS_SWEEP_LEARNED_NARROW 512, 32
