@@ -144,6 +144,7 @@ typedef struct ts_learned_cfg {
   uint32_t iters;            /* optimiser steps; default 40 */
   float lr;                  /* Adam step size; default 0.02 */
   uint32_t points;           /* training points taken from the sorted sample; default 16384 */
+  uint32_t verify;           /* 1: judge the output (see below); default 1 */
 } ts_learned_cfg;
 
 typedef struct ts_learned_report {
@@ -156,8 +157,15 @@ typedef struct ts_learned_report {
   float loss_before;         /* mean squared CDF error on the training points: the histogram start */
   float loss_after;          /* ... after the optimiser steps (the model used) */
   float loss_uniform;        /* ... of uniform knots (the untrained model) */
+  uint32_t verified;         /* 1: the judge ran and passed */
 } ts_learned_report;
 
+/* With cfg->verify (the default) ts_learned_sort judges its own output before
+ * returning: the input's multiset fingerprint is taken on the device before
+ * the partition, and the output must hold the same fingerprint with no
+ * adjacent pair out of order (ts_verify_sorted); otherwise the call returns
+ * TS_ERR_VERIFY_FAILED.  Device errors recorded during the sort (see
+ * ts_ctx_check) also fail it. */
 void ts_learned_cfg_default(ts_learned_cfg* cfg);
 ts_status ts_learned_sort(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, const ts_learned_cfg* cfg,
                           ts_learned_report* report, void* stream);
@@ -193,6 +201,16 @@ const char* ts_sbk_last_error(const ts_sbk* b);
  *   kind 2: u64 = zero-extended high 32 bits. */
 ts_status ts_fill_splitmix64(ts_ctx* ctx, void* d_out, size_t n, uint64_t seed, int kind, void* stream);
 
+/* The output judge: TS_OK when no adjacent pair of the n keys at d_data is
+ * out of order in the type's total order and, if expect_sum_xor is not NULL,
+ * their multiset fingerprint (as ts_checksum computes it) equals
+ * expect_sum_xor; else TS_ERR_VERIFY_FAILED, with ts_ctx_last_error saying
+ * which.  Waits for the device.  The closest reference check is the
+ * adjacent-pair order scan of presorted.rs:35-65 (all_pairs); the
+ * fingerprint has no counterpart there. */
+ts_status ts_verify_sorted(ts_ctx* ctx, ts_dtype dtype, const void* d_data, size_t n,
+                           const uint64_t* expect_sum_xor, void* stream);
+
 /* Order-independent multiset checksum of n keys of `dtype` (sum and xor of a
  * 64-bit mix of each key's bits) and the count of adjacent pairs out of order
  * in the type's total order.  Synchronous: results land in host memory. */
@@ -224,6 +242,51 @@ int ts_ctx_ranking(const ts_ctx* ctx);
  * Any value gives the same output. */
 ts_status ts_ctx_set_split_rounds(ts_ctx* ctx, int rounds);
 int ts_ctx_split_rounds(const ts_ctx* ctx);
+
+/* Sorts of 8-byte keys without values (ts_sort, ts_sort_with_buffer,
+ * ts_sort_host) of at least this many keys (default 2^23; SIZE_MAX: never)
+ * divert as the reference's byte-digit core does (radix/mod.rs:240-242,
+ * 268-270; divert.rs:38-90): only the top live digits that carry about
+ * log2(n) + 2 bits are sorted, decided on the device from the histograms,
+ * and one finishing read sorts each run of keys that share them.  A run
+ * longer than the finish sorts that is out of order makes the device sort
+ * every digit after all.  Any value gives the same output; sizes at or
+ * below 2^21 keys never divert. */
+ts_status ts_ctx_set_divert_min(ts_ctx* ctx, size_t n);
+size_t ts_ctx_divert_min(const ts_ctx* ctx);
+
+/* Device-side checks.  Kernels record what they cannot return in a sticky
+ * error word of the context: a look-back wait that hit its bound (the sort's
+ * output is wrong; never expected), a segment longer than the max_segment_len
+ * given to an asynchronous ts_segmented_sort_pairs (left unsorted), and
+ * invalid segment offsets in such a call.  ts_ctx_check waits for `stream`,
+ * reads and clears the word, and returns TS_ERR_VERIFY_FAILED (with
+ * ts_ctx_last_error naming each cause) when it was set.  ts_sort_host,
+ * ts_learned_sort and the waiting forms of ts_segmented_sort_pairs check it
+ * themselves before returning. */
+ts_status ts_ctx_check(ts_ctx* ctx, void* stream);
+
+/* What the last radix sort on this context did (waits for `stream`): live
+ * digits, scatter passes executed (by the second sort, when it ran), digits
+ * the divert left to its finish and the bit the finish started from (0 after
+ * a second sort), whether the second sort ran, and whether non-increasing
+ * input was reversed instead of sorted. */
+typedef struct ts_sort_info {
+  uint32_t live_digits;
+  uint32_t executed_passes;
+  uint32_t skipped_digits;
+  uint32_t finish_shift;
+  uint32_t fallback;
+  uint32_t reversed;
+} ts_sort_info;
+ts_status ts_ctx_last_sort_info(ts_ctx* ctx, ts_sort_info* info, void* stream);
+
+/* Workspace planning, host arithmetic only (touches no device): bytes of
+ * look-back status one radix sort of n keys of `dtype` uses with scatter
+ * tiles of `kind` (0 keys only, 1 keys with u32 order values, 2 learned
+ * bucket digits), or, for kind -1, the bytes a context reserves so that every
+ * radix sort of at most n keys of that width fits, whatever its kind. */
+size_t ts_plan_status_bytes(ts_dtype dtype, size_t n, int kind);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,6 @@ def test_learned_defaults_agree_across_layers():
     c = _native.LearnedCfg()
     L.ts_learned_cfg_default(ctypes.byref(c))
     py = ts.LearnedConfig()
-    assert (c.buckets_log2, c.sample_per_mille, bool(c.train), c.iters, c.points) == (
-        py.buckets_log2, py.sample_per_mille, py.train, py.iters, py.points)
+    assert (c.buckets_log2, c.sample_per_mille, bool(c.train), c.iters, c.points, bool(c.verify)) == (
+        py.buckets_log2, py.sample_per_mille, py.train, py.iters, py.points, py.verify)
     assert abs(c.lr - py.lr) < 1e-7

@@ -26,6 +26,8 @@ def check_sorted(x: np.ndarray, dev, config=None) -> dict:
     rep = ts.learned_sort(t, config=config)
     torch.cuda.synchronize()
     assert np.array_equal(bits(to_host(t, x.dtype)), bits(cf.sort(x)))
+    if x.size > 1 and (config is None or config.verify):
+        assert rep["verified"] == 1, rep  # the output judge ran and passed
     return rep
 
 
@@ -145,3 +147,39 @@ def test_heavy_hitters_take_the_split_path(gpu, dtype):
     x = np.where(pick < 0.5, hot[0], np.where(pick < 0.7, hot[1], np.where(pick < 0.75, hot[2], x)))
     rep = check_sorted(x.astype(cf.NP_DTYPES[dtype]), gpu)
     assert rep["overflow_buckets"] >= 1 and rep["split_rounds"] >= 1, rep
+
+
+@pytest.mark.parametrize("per_mille", [10, 600])
+def test_config_sample_sizes(gpu, per_mille):
+    """The workload's 1 % sample, and a sample above half the keys (its own
+    radix sort then runs tiles the partition's status rows were not sized
+    for alone)."""
+    cfg = ts.LearnedConfig(sample_per_mille=per_mille)
+    for n in (1_000_000, 3_000_017):
+        rep = check_sorted(cf.lognormal_mixture_keys(8, n), gpu, cfg)
+        assert rep["sample"] == n * per_mille // 1000
+
+
+def test_judge_rejects_a_corrupted_output(gpu):
+    """ts_verify_sorted, the learned sort's judge, on outputs corrupted from
+    the host: a swapped adjacent pair fails the order check, a changed key
+    the multiset fingerprint; the true output passes both."""
+    x = cf.lognormal_mixture_keys(8, 1_000_000)
+    want = cf.sort(x)
+    fp = ts.checksum(to_dev(x, gpu))[:2]
+    ts.verify_sorted(to_dev(want, gpu), fp)
+    bad = want.copy()
+    bad[[5000, 5001]] = bad[[5001, 5000]]
+    if bad[5000] != bad[5001]:
+        with pytest.raises(RuntimeError, match="out of order"):
+            ts.verify_sorted(to_dev(bad, gpu), fp)
+    changed = want.copy()
+    changed[-1] = changed[-1] + np.uint64(1)
+    ts.verify_sorted(to_dev(changed, gpu))  # still sorted
+    with pytest.raises(RuntimeError, match="multiset"):
+        ts.verify_sorted(to_dev(changed, gpu), fp)
+
+
+def test_learned_sort_without_the_judge(gpu):
+    rep = check_sorted(cf.lognormal_mixture_keys(9, 500_000), gpu, ts.LearnedConfig(verify=False))
+    assert rep["verified"] == 0

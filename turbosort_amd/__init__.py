@@ -125,6 +125,30 @@ class Context:
     def set_split_rounds(self, rounds: int) -> None:
         self.check(self._lib.ts_ctx_set_split_rounds(self._h, int(rounds)))
 
+    @property
+    def divert_min(self) -> int:
+        """Keys from which sorts of 8-byte keys without values divert (sort
+        only the top digits that carry about log2(n) + 2 bits, then finish the
+        runs, as the reference's radix/mod.rs:240-242 does)."""
+        return int(self._lib.ts_ctx_divert_min(self._h))
+
+    def set_divert_min(self, n: int) -> None:
+        self.check(self._lib.ts_ctx_set_divert_min(self._h, int(n)))
+
+    def check_device(self) -> None:
+        """Wait for this context's stream work and raise if a kernel recorded
+        an error (a look-back wait that hit its bound, a segment longer than a
+        given max_segment_len, invalid segment offsets); clears the record."""
+        self.check(self._lib.ts_ctx_check(self._h, _stream(self.device)))
+
+    def last_sort_info(self) -> dict:
+        """What the last radix sort on this context did: live digits, passes
+        executed, digits left to the divert's finish, whether the divert's
+        second sort ran, whether the input was reversed (waits for the device)."""
+        info = _native.SortInfo()
+        self.check(self._lib.ts_ctx_last_sort_info(self._h, ctypes.byref(info), _stream(self.device)))
+        return {name: int(getattr(info, name)) for name, _ in info._fields_}
+
     def kernel_times(self):
         """[(kernel name, ms, algorithmic bytes)] of the last call, from HIP events."""
         cap = 256
@@ -387,6 +411,7 @@ class LearnedConfig:
     iters: int = 40
     lr: float = 0.02
     points: int = 16384
+    verify: bool = True
 
 
 def learned_sort(t: torch.Tensor, *, config: Optional[LearnedConfig] = None, ctx: Optional[Context] = None) -> dict:
@@ -400,7 +425,8 @@ def learned_sort(t: torch.Tensor, *, config: Optional[LearnedConfig] = None, ctx
     _slice(t)
     code = _dtype_code(t.dtype)
     c = config or LearnedConfig()
-    cfg = _native.LearnedCfg(c.buckets_log2, c.sample_per_mille, int(bool(c.train)), c.iters, c.lr, c.points)
+    cfg = _native.LearnedCfg(c.buckets_log2, c.sample_per_mille, int(bool(c.train)), c.iters, c.lr, c.points,
+                             int(bool(c.verify)))
     rep = _native.LearnedReport()
     ctx = ctx or _default_context(t.device)
     ctx.check(ctx._lib.ts_learned_sort(ctx.handle, code, ctypes.c_void_p(t.data_ptr()), t.numel(), ctypes.byref(cfg),
@@ -455,6 +481,19 @@ def fill_splitmix64_zero_extended(out: torch.Tensor, seed: int, *, ctx: Optional
     ctx = ctx or _default_context(out.device)
     ctx.check(ctx._lib.ts_fill_splitmix64(ctx.handle, ctypes.c_void_p(out.data_ptr()), out.numel(), seed, 2,
                                           _stream(out.device)))
+
+
+def verify_sorted(t: torch.Tensor, expect=None, *, ctx: Optional[Context] = None) -> None:
+    """The output judge (ts_verify_sorted): raises ``RuntimeError`` unless no
+    adjacent pair of ``t`` is out of order and, when ``expect`` = (sum, xor)
+    of ``checksum`` on the input is given, ``t`` holds the same multiset."""
+    _slice(t)
+    ctx = ctx or _default_context(t.device)
+    exp = None
+    if expect is not None:
+        exp = (ctypes.c_uint64 * 2)(int(expect[0]), int(expect[1]))
+    ctx.check(ctx._lib.ts_verify_sorted(ctx.handle, _dtype_code(t.dtype), ctypes.c_void_p(t.data_ptr()), t.numel(),
+                                        exp, _stream(t.device)))
 
 
 def checksum(t: torch.Tensor, *, ctx: Optional[Context] = None):

@@ -21,6 +21,8 @@ EXPORTS = [
     "ts_learned_cfg_default", "ts_learned_sort", "ts_pwl_fit",
     "ts_segmented_variants", "ts_segmented_variant_name", "ts_segmented_set_variant",
     "ts_ctx_set_ranking", "ts_ctx_ranking", "ts_sort_host", "ts_ctx_set_split_rounds", "ts_ctx_split_rounds",
+    "ts_ctx_set_divert_min", "ts_ctx_divert_min", "ts_ctx_check", "ts_ctx_last_sort_info", "ts_plan_status_bytes",
+    "ts_verify_sorted",
 ]
 
 
@@ -28,14 +30,22 @@ class LearnedCfg(ctypes.Structure):
     """ts_learned_cfg"""
     _fields_ = [("buckets_log2", ctypes.c_uint32), ("sample_per_mille", ctypes.c_uint32),
                 ("train", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("lr", ctypes.c_float),
-                ("points", ctypes.c_uint32)]
+                ("points", ctypes.c_uint32), ("verify", ctypes.c_uint32)]
 
 
 class LearnedReport(ctypes.Structure):
     """ts_learned_report"""
     _fields_ = [("sample", ctypes.c_uint64), ("buckets", ctypes.c_uint32), ("overflow_buckets", ctypes.c_uint32),
                 ("split_rounds", ctypes.c_uint32), ("overflow_keys", ctypes.c_uint64), ("largest_bucket", ctypes.c_uint64),
-                ("loss_before", ctypes.c_float), ("loss_after", ctypes.c_float), ("loss_uniform", ctypes.c_float)]
+                ("loss_before", ctypes.c_float), ("loss_after", ctypes.c_float), ("loss_uniform", ctypes.c_float),
+                ("verified", ctypes.c_uint32)]
+
+class SortInfo(ctypes.Structure):
+    """ts_sort_info"""
+    _fields_ = [("live_digits", ctypes.c_uint32), ("executed_passes", ctypes.c_uint32),
+                ("skipped_digits", ctypes.c_uint32), ("finish_shift", ctypes.c_uint32),
+                ("fallback", ctypes.c_uint32), ("reversed", ctypes.c_uint32)]
+
 
 TS_OK = 0
 TS_ERR_BUFFER_TOO_SMALL = 1
@@ -112,6 +122,18 @@ def load() -> ctypes.CDLL:
     L.ts_ctx_set_split_rounds.restype = i32
     L.ts_ctx_split_rounds.argtypes = [vp]
     L.ts_ctx_split_rounds.restype = i32
+    L.ts_ctx_set_divert_min.argtypes = [vp, sz]
+    L.ts_ctx_set_divert_min.restype = i32
+    L.ts_ctx_divert_min.argtypes = [vp]
+    L.ts_ctx_divert_min.restype = sz
+    L.ts_ctx_check.argtypes = [vp, vp]
+    L.ts_ctx_check.restype = i32
+    L.ts_ctx_last_sort_info.argtypes = [vp, ctypes.POINTER(SortInfo), vp]
+    L.ts_ctx_last_sort_info.restype = i32
+    L.ts_plan_status_bytes.argtypes = [i32, sz, i32]
+    L.ts_plan_status_bytes.restype = sz
+    L.ts_verify_sorted.argtypes = [vp, i32, vp, sz, ctypes.POINTER(u64), vp]
+    L.ts_verify_sorted.restype = i32
     L.ts_ctx_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_double), i32]
     L.ts_ctx_kernel_times.restype = i32

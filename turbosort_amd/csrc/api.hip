@@ -45,9 +45,21 @@ struct ts_ctx {
   // (grow-only), so those copies are direct transfers.
   char* pin = nullptr;
   size_t pin_bytes = 0;
+  // Device block of 256 bytes: the sticky error word (kErr* bits, read and
+  // cleared by ts_ctx_check and by the calls that wait for the device
+  // anyway) at byte 0, then two fingerprints of three words each (input and
+  // output of a judged sort) at kFpIn and kFpOut.
+  uint32_t* dflags = nullptr;
+  // Sorts of 8-byte keys without values divert from this many keys on.
+  size_t divert_min = size_t(1) << 23;
 };
 
 namespace {
+
+constexpr size_t kFpIn = 64, kFpOut = 128;  // byte offsets in ts_ctx::dflags
+unsigned long long* fp_slot(const ts_ctx* c, size_t off) {
+  return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(c->dflags) + off);
+}
 
 constexpr size_t kAlign = 256;
 inline size_t up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
@@ -113,6 +125,61 @@ ts_status ensure_pinned(ts_ctx* c, size_t bytes) {
 
 bool valid_dtype(int dt) { return dt >= TS_U8 && dt <= TS_F64; }
 
+// What a set of sticky error bits means, for ts_ctx_last_error.
+std::string flag_text(uint32_t f) {
+  std::string m = "device check failed:";
+  if (f & ts::kErrLookback) m += " a look-back wait hit its bound (the sort's output is wrong);";
+  if (f & ts::kErrSegHint) m += " a segment was longer than max_segment_len (any beyond one workgroup's capacity was left unsorted);";
+  if (f & ts::kErrSegOffsets) m += " segment offsets were decreasing or past n (those segments were skipped);";
+  return m;
+}
+
+// Fingerprints data into fp_slot(c, kFpOut), waits, and compares it with
+// `expect` (sum, xor; null: sortedness only).
+ts_status verify_output(ts_ctx* c, int kb, int mode, const void* data, size_t n, const uint64_t* expect,
+                        hipStream_t s) {
+  unsigned long long* out = fp_slot(c, kFpOut);
+  hipError_t e = ts::checksum(kb, mode, data, n, out, s, &c->clk);
+  unsigned long long h[3] = {0, 0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(c, e, "verify");
+  if (h[2] != 0)
+    return fail(c, TS_ERR_VERIFY_FAILED,
+                "output not sorted: " + std::to_string(h[2]) + " adjacent pairs out of order");
+  if (expect && (h[0] != expect[0] || h[1] != expect[1]))
+    return fail(c, TS_ERR_VERIFY_FAILED, "output keys differ from the input's (multiset fingerprint)");
+  return TS_OK;
+}
+
+// Waits for stream s, then reads and clears the context's sticky error word.
+ts_status check_flags(ts_ctx* c, hipStream_t s) {
+  uint32_t f = 0;
+  hipError_t e = hipMemcpyAsync(&f, c->dflags, sizeof f, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(c, e, "error word readback");
+  if (f == 0) return TS_OK;
+  e = hipMemsetAsync(c->dflags, 0, sizeof f, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(c, e, "error word clear");
+  return fail(c, TS_ERR_VERIFY_FAILED, flag_text(f));
+}
+
+// A radix job on the context's device with its sticky error word.
+ts::RadixJob radix_job(const ts_ctx* c, int kb, int mode, size_t n, char* meta, char* status, size_t status_cap) {
+  ts::RadixJob j;
+  j.key_bytes = kb;
+  j.mode = mode;
+  j.n = n;
+  j.meta = reinterpret_cast<ts::SortMeta*>(meta);
+  j.status = status;
+  j.status_cap = status_cap;
+  j.error = c->dflags;
+  j.num_cus = c->num_cus;
+  j.atomic_rank = c->atomic_rank;
+  return j;
+}
+
 // Workspace layouts.  meta | look-back status | arrays.
 size_t sort_ws(size_t n, int kb, bool own_scratch) {
   size_t b = up(sizeof(ts::SortMeta));
@@ -145,18 +212,12 @@ ts_status sort_impl(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, hip
   ts_status st = ensure(c, sort_ws(n, kb, buf == nullptr));
   if (st != TS_OK) return st;
   char* p = c->ws;
-  ts::RadixJob j;
-  j.key_bytes = kb;
-  j.mode = mode;
-  j.n = n;
-  j.meta = reinterpret_cast<ts::SortMeta*>(p);
-  p += up(sizeof(ts::SortMeta));
-  j.status = p;
-  p += up(ts::status_bytes(n, kb));
+  const size_t sb = ts::status_bytes(n, kb);
+  ts::RadixJob j = radix_job(c, kb, mode, n, p, p + up(sizeof(ts::SortMeta)), sb);
+  p += up(sizeof(ts::SortMeta)) + up(sb);
   j.keys_a = data;
   j.keys_b = buf ? buf : p;
-  j.num_cus = c->num_cus;
-  j.atomic_rank = c->atomic_rank;
+  j.divert = kb == 8 && n >= c->divert_min;
   hipError_t e = ts::radix_sort(j, s, &c->clk);
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort");
 }
@@ -168,16 +229,10 @@ ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint
   if (!vals && m > ts::kSmallMax) {  // keys only: the plain device sort in place
     ts_status st = ensure(c, sort_ws(m, kb, true));
     if (st != TS_OK) return st;
-    ts::RadixJob j;
-    j.key_bytes = kb;
-    j.mode = mode;
-    j.n = m;
-    j.meta = reinterpret_cast<ts::SortMeta*>(c->ws);
-    j.status = c->ws + up(sizeof(ts::SortMeta));
+    const size_t sb = ts::status_bytes(m, kb);
+    ts::RadixJob j = radix_job(c, kb, mode, m, c->ws, c->ws + up(sizeof(ts::SortMeta)), sb);
     j.keys_a = keys;
-    j.keys_b = c->ws + up(sizeof(ts::SortMeta)) + up(ts::status_bytes(m, kb));
-    j.num_cus = c->num_cus;
-    j.atomic_rank = c->atomic_rank;
+    j.keys_b = c->ws + up(sizeof(ts::SortMeta)) + up(sb);
     hipError_t e = ts::radix_sort(j, s, &c->clk);
     return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort");
   }
@@ -194,17 +249,12 @@ ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint
     e = ts::small_sort(kb, mode, nullptr, m, keys, size_t(kb), 0, order, s, &c->clk);
     if (e != hipSuccess) return hip_fail(c, e, "small_sort");
   } else {
-    ts::RadixJob j;
-    j.key_bytes = kb;
-    j.mode = mode;
-    j.n = m;
+    const size_t sb = ts::status_bytes(m, kb);
+    ts::RadixJob j = radix_job(c, kb, mode, m, p, p + up(sizeof(ts::SortMeta)), sb);
     j.records = keys;
     j.stride = size_t(kb);
     j.key_offset = 0;
-    j.meta = reinterpret_cast<ts::SortMeta*>(p);
-    p += up(sizeof(ts::SortMeta));
-    j.status = p;
-    p += up(ts::status_bytes(m, kb));
+    p += up(sizeof(ts::SortMeta)) + up(sb);
     j.keys_a = p;
     p += up(m * size_t(kb));
     j.keys_b = p;
@@ -217,8 +267,6 @@ ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint
     p += up(m * 4);
     gbuf = p;
     j.final_vals = order;
-    j.num_cus = c->num_cus;
-    j.atomic_rank = c->atomic_rank;
     e = ts::radix_sort(j, s, &c->clk);
     if (e != hipSuccess) return hip_fail(c, e, "radix_sort");
   }
@@ -296,16 +344,10 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
   const uint32_t kRounds = uint32_t(c->split_rounds);
   const uint32_t cap = ts::seg_lds_max_len(kb, false);
   auto radix_part = [&](uint64_t lo, uint64_t len) {
-    ts::RadixJob j;
-    j.key_bytes = kb;
-    j.mode = mode;
-    j.n = len;
-    j.meta = reinterpret_cast<ts::SortMeta*>(radix_ws);
-    j.status = radix_ws + up(sizeof(ts::SortMeta));
+    // radix_ws was sized for n keys, which covers every part of at most n.
+    ts::RadixJob j = radix_job(c, kb, mode, len, radix_ws, radix_ws + up(sizeof(ts::SortMeta)), ts::status_bytes(n, kb));
     j.keys_a = static_cast<unsigned char*>(data) + lo * size_t(kb);
     j.keys_b = static_cast<unsigned char*>(tmp) + lo * size_t(kb);
-    j.num_cus = c->num_cus;
-    j.atomic_rank = c->atomic_rank;
     return ts::radix_sort(j, s, &c->clk);
   };
   size_t o = 0;
@@ -405,6 +447,7 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
       gj.lists = d_lists;
       gj.num_cus = c->num_cus;
       gj.atomic_rank = c->atomic_rank;
+      gj.error = c->dflags;
       set_variants(c, kb, false, gj);
       if (e == hipSuccess) e = ts::segmented_sort(gj, s, &c->clk);
       if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
@@ -417,8 +460,8 @@ ts_status sort_long_keys(ts_ctx* c, int kb, int mode, void* data, void* tmp, siz
   for (const auto& sg : segs)
     if (e == hipSuccess) e = radix_part(sg.first, sg.second);
   // The last parts' copies read pinned memory that the next call may rewrite.
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  return e == hipSuccess ? TS_OK : hip_fail(c, e, "split rounds");
+  if (e != hipSuccess) return hip_fail(c, e, "split rounds");
+  return check_flags(c, s);
 }
 
 }  // namespace
@@ -478,6 +521,16 @@ ts_status ts_ctx_create(ts_ctx** out, int device) {
   c->device = device;
   c->atomic_rank_ok = ok;
   c->atomic_rank = ok;
+  {
+    DeviceScope ds(device);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&c->dflags), 256);
+    if (e == hipSuccess) e = hipMemset(c->dflags, 0, 256);
+    if (e != hipSuccess) {
+      if (c->dflags) (void)hipFree(c->dflags);
+      delete c;
+      return TS_ERR_HIP;
+    }
+  }
   // Keys-only 4-byte segments of the largest class: the MSD shape by
   // default (8-byte keys have no other shape for that class).
   c->seg_variant[2][0][8] = 2;
@@ -512,10 +565,61 @@ ts_status ts_ctx_set_split_rounds(ts_ctx* c, int rounds) {
 
 int ts_ctx_split_rounds(const ts_ctx* c) { return c ? c->split_rounds : -1; }
 
+ts_status ts_ctx_set_divert_min(ts_ctx* c, size_t n) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  c->divert_min = n;
+  return TS_OK;
+}
+
+size_t ts_ctx_divert_min(const ts_ctx* c) { return c ? c->divert_min : 0; }
+
+ts_status ts_ctx_check(ts_ctx* c, void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  DeviceScope ds(c->device);
+  return check_flags(c, static_cast<hipStream_t>(stream));
+}
+
+ts_status ts_ctx_last_sort_info(ts_ctx* c, ts_sort_info* info, void* stream) {
+  if (!c || !info) return TS_ERR_INVALID_ARG;
+  std::memset(info, 0, sizeof *info);
+  if (!c->ws || c->ws_bytes < sizeof(ts::SortMeta)) return TS_OK;
+  DeviceScope ds(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const ts::SortMeta* m = reinterpret_cast<const ts::SortMeta*>(c->ws);
+  uint32_t w[8];
+  hipError_t e = hipMemcpyAsync(&w[0], &m->all_live, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&w[1], &m->n_live, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&w[2], &m->diverted, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&w[3], &m->finish_shift, 8, hipMemcpyDeviceToHost, s);  // shift, fallback
+  if (e == hipSuccess) e = hipMemcpyAsync(&w[5], &m->reverse, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(c, e, "sort info readback");
+  info->live_digits = w[0];
+  info->executed_passes = w[1];
+  info->skipped_digits = w[2];
+  info->finish_shift = w[3];
+  info->fallback = w[4];
+  info->reversed = w[5];
+  return TS_OK;
+}
+
+size_t ts_plan_status_bytes(ts_dtype dt, size_t n, int kind) {
+  if (!valid_dtype(dt)) return 0;
+  const int kb = ts::dtype_bytes(dt);
+  switch (kind) {
+    case -1: return ts::status_bytes(n, kb);
+    case ts::kSweepKeys: return ts::status_bytes_used(n, kb, false, false);
+    case ts::kSweepVals: return ts::status_bytes_used(n, kb, false, true);
+    case ts::kSweepLearned: return kb >= 4 ? ts::status_bytes_used(n, kb, true, false) : 0;
+    default: return 0;
+  }
+}
+
 void ts_ctx_destroy(ts_ctx* c) {
   if (!c) return;
   DeviceScope ds(c->device);
   if (c->ws) hipFree(c->ws);
+  if (c->dflags) hipFree(c->dflags);
   if (c->hdev) hipFree(c->hdev);
   if (c->pin) hipHostFree(c->pin);
   if (c->aux_done) hipEventDestroy(c->aux_done);
@@ -594,8 +698,7 @@ ts_status ts_sort_host(ts_ctx* c, ts_dtype dt, void* host_data, size_t n, void* 
     hipError_t e = hipMemcpyAsync(h + o, c->hdev + o, std::min(kChunk, bytes - o), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(c, e, "hipMemcpyAsync(device to host)");
   }
-  hipError_t e = hipStreamSynchronize(s);
-  return e == hipSuccess ? TS_OK : hip_fail(c, e, "hipStreamSynchronize");
+  return check_flags(c, s);
 }
 
 ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t stride, size_t key_offset,
@@ -630,17 +733,12 @@ ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t
     e = ts::small_sort(kb, mode, nullptr, n, rec, stride, key_offset, order, s, &c->clk);
     if (e != hipSuccess) return hip_fail(c, e, "small_sort");
   } else {
-    ts::RadixJob j;
-    j.key_bytes = kb;
-    j.mode = mode;
-    j.n = n;
+    const size_t sb = ts::status_bytes(n, kb);
+    ts::RadixJob j = radix_job(c, kb, mode, n, p, p + up(sizeof(ts::SortMeta)), sb);
     j.records = rec;
     j.stride = stride;
     j.key_offset = key_offset;
-    j.meta = reinterpret_cast<ts::SortMeta*>(p);
-    p += up(sizeof(ts::SortMeta));
-    j.status = p;
-    p += up(ts::status_bytes(n, kb));
+    p += up(sizeof(ts::SortMeta)) + up(sb);
     j.keys_a = p;
     p += up(n * size_t(kb));
     j.keys_b = p;
@@ -650,8 +748,6 @@ ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t
     j.vals_b = reinterpret_cast<uint32_t*>(p);
     p += up(n * 4);
     j.final_vals = order;
-    j.num_cus = c->num_cus;
-    j.atomic_rank = c->atomic_rank;
     gbuf = p;
     e = ts::radix_sort(j, s, &c->clk);
     if (e != hipSuccess) return hip_fail(c, e, "radix_sort");
@@ -718,10 +814,15 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
   j.lists = reinterpret_cast<uint32_t*>(c->ws + count_bytes);
   j.num_cus = c->num_cus;
   j.atomic_rank = c->atomic_rank;
+  j.error = c->dflags;
+  const bool async = max_segment_len != 0 && max_segment_len <= ts::seg_lds_max_len(kb, vals != nullptr);
+  // Without a host wait, a violated bound or bad offsets are recorded in the
+  // sticky error word for the next ts_ctx_check (or call that waits).
+  j.max_len = async ? max_segment_len : 0;
   set_variants(c, kb, vals != nullptr, j);
   hipError_t e = ts::segmented_sort(j, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "segmented_sort");
-  if (max_segment_len != 0 && max_segment_len <= ts::seg_lds_max_len(kb, vals != nullptr)) return TS_OK;
+  if (async) return TS_OK;
 
   // Unknown or long segments: read the class counts and the long segments'
   // bounds back (one copy unless they are many).
@@ -734,9 +835,11 @@ ts_status ts_segmented_sort_pairs(ts_ctx* c, ts_dtype dt, void* keys, uint32_t* 
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e == hipSuccess) e = finish_long_bounds(c, back, fmax, s, &fat);
   if (e != hipSuccess) return hip_fail(c, e, "segment bounds");
-  if (reinterpret_cast<const uint32_t*>(c->pin)[ts::kSegClasses])
+  if (reinterpret_cast<const uint32_t*>(c->pin)[ts::kSegClasses]) {
+    (void)check_flags(c, s);  // the bit this call set is answered here
     return fail(c, TS_ERR_INVALID_ARG, "segment offsets must be non-decreasing and at most n");
-  if (fat.empty()) return TS_OK;
+  }
+  if (fat.empty()) return check_flags(c, s);
   if (!vals) {
     // Keys only: three-way splits, then the LDS classes (the workspace's
     // class lists are no longer needed, so it may be regrown).
@@ -762,6 +865,7 @@ void ts_learned_cfg_default(ts_learned_cfg* cfg) {
   cfg->iters = 40;
   cfg->lr = 0.02f;
   cfg->points = 16384;
+  cfg->verify = 1;
 }
 
 ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts_learned_cfg* cfg_in,
@@ -794,8 +898,11 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   size_t off[17];
   size_t b = 0;
   auto take = [&](int i, size_t bytes) { off[i] = b; b += up(bytes); };
+  // The look-back status serves the sample's sort (8-byte keys) and the
+  // partition.
+  const size_t sb = std::max(ts::status_bytes(n, kb), ts::status_bytes(S, 8));
   take(0, sizeof(ts::SortMeta));
-  take(1, ts::status_bytes(n, kb, true));
+  take(1, sb);
   take(2, n * size_t(kb));
   take(3, S * 8);
   take(4, S * 8);
@@ -842,9 +949,14 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   j.offsets = reinterpret_cast<unsigned long long*>(w + off[11]);
   j.meta = reinterpret_cast<ts::SortMeta*>(w + off[0]);
   j.status = w + off[1];
+  j.status_cap = sb;
+  j.error = c->dflags;
   j.num_cus = c->num_cus;
   j.atomic_rank = c->atomic_rank;
-  hipError_t e = ts::learned_partition(j, s, &c->clk);
+  // The judge's reference: the input keys' multiset fingerprint.
+  hipError_t e = hipSuccess;
+  if (cfg.verify) e = ts::checksum(kb, mode, data, n, fp_slot(c, kFpIn), s, &c->clk);
+  if (e == hipSuccess) e = ts::learned_partition(j, s, &c->clk);
   if (e != hipSuccess) return hip_fail(c, e, "learned_partition");
 
   ts::SegJob sj;
@@ -858,6 +970,7 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   sj.lists = reinterpret_cast<uint32_t*>(w + off[13]);
   sj.num_cus = c->num_cus;
   sj.atomic_rank = c->atomic_rank;
+  sj.error = c->dflags;
   set_variants(c, kb, false, sj);
   sj.stage = 1;  // classify only: the readback block is queued between the classify and the bucket sort
   e = ts::segmented_sort(sj, s, &c->clk);
@@ -904,6 +1017,21 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
     if (e == hipSuccess) e = hipStreamWaitEvent(s, c->aux_done, 0);
     if (e != hipSuccess) return hip_fail(c, e, "auxiliary stream join");
   }
+  uint32_t verified = 0;
+  if (cfg.verify) {
+    // The judge: every output key in order, and the same multiset as the
+    // input's (fingerprint taken before the partition).
+    uint64_t want[3] = {0, 0, 0};
+    e = hipMemcpyAsync(want, fp_slot(c, kFpIn), sizeof want, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(c, e, "verify");
+    // verify_output reads `want` after waiting for the stream, by when the
+    // copy above has landed.
+    st = verify_output(c, kb, mode, data, n, want, s);
+    if (st != TS_OK) return st;
+    verified = 1;
+  }
+  st = check_flags(c, s);
+  if (st != TS_OK) return st;
   if (report) {
     report->sample = S;
     report->buckets = B;
@@ -914,8 +1042,19 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
     report->loss_before = loss[0];
     report->loss_after = loss[1];
     report->loss_uniform = loss[2];
+    report->verified = verified;
   }
   return TS_OK;
+}
+
+ts_status ts_verify_sorted(ts_ctx* c, ts_dtype dt, const void* data, size_t n, const uint64_t* expect_sum_xor,
+                           void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(dt)) return fail(c, TS_ERR_INVALID_ARG, "unknown dtype");
+  if (n && !data) return fail(c, TS_ERR_INVALID_ARG, "null data");
+  DeviceScope ds(c->device);
+  return verify_output(c, ts::dtype_bytes(dt), ts::dtype_mode(dt), data, n, expect_sum_xor,
+                       static_cast<hipStream_t>(stream));
 }
 
 ts_status ts_pwl_fit(ts_ctx* c, const float* u, const float* t, size_t T, float* theta, float* grad, float* loss2,
@@ -1039,7 +1178,7 @@ ts_status ts_checksum(ts_ctx* c, ts_dtype dt, const void* data, size_t n, uint64
   ts_status st = ensure(c, up(sizeof(ts::SortMeta)));
   if (st != TS_OK) return st;
   ts::SortMeta* m = reinterpret_cast<ts::SortMeta*>(c->ws);
-  hipError_t e = ts::checksum(ts::dtype_bytes(dt), ts::dtype_mode(dt), data, n, m, s);
+  hipError_t e = ts::checksum(ts::dtype_bytes(dt), ts::dtype_mode(dt), data, n, m->check, s);
   if (e != hipSuccess) return hip_fail(c, e, "checksum");
   unsigned long long h[4];
   e = hipMemcpyAsync(h, m->check, sizeof(h), hipMemcpyDeviceToHost, s);

@@ -83,50 +83,15 @@ __host__ __device__ __forceinline__ int dtype_mode(int dt) {
 // no host round trip.
 //
 // Each scatter pass splits its input into kSegments contiguous segments and
-// runs one decoupled look-back chain per segment (tickets interleave the
-// segments, so every chain advances at a fraction of the tile rate).  The
-// first pass reads the input in its original order and uses position
-// segments; a later pass reads keys ordered by the previous digit and uses
-// value segments, the top three bits of the previous digit.  pair[p][s][x]
-// counts the keys of segment s (of digit p's pass) whose digit p is x; it is
-// filled in the single histogram read, and gives the digit bases, each
-// segment's offset inside each digit's run (segpre) and the segment bounds.
+// runs one decoupled look-back chain per segment (each chain is claimed by
+// the workgroups of one XCD first, so it advances at a fraction of the tile
+// rate).  The first executed pass reads the input in its original order and
+// uses position segments; a later pass reads keys ordered by the previous
+// digit and uses value segments, the top three bits of the previous digit.
+// The segmented digit tables are filled in the single histogram read, and
+// give the digit bases, each segment's first output position for each digit
+// value (pbase) and the segment bounds.
 constexpr int kSegments = 8;
-
-// Ticket order of a pass's tiles: round t holds tile t of every segment that
-// has more than t tiles, in segment order, and precedes round t+1.  So a
-// tile's predecessor in its segment always has a smaller ticket.  Maps
-// ticket k (< the total tile count) to (segment, round).
-__host__ __device__ __forceinline__ void ticket_to_tile(uint32_t k, const uint32_t (&nt)[kSegments], uint32_t* seg,
-                                                        uint32_t* round) {
-  uint32_t most = 0;
-  for (int s = 0; s < kSegments; ++s) most = nt[s] > most ? nt[s] : most;
-  // Largest t with F(t) <= k, F(t) = sum_s min(nt[s], t) (tiles in rounds < t).
-  uint32_t lo = 0, hi = most;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    uint32_t f = 0;
-    for (int s = 0; s < kSegments; ++s) f += nt[s] < mid ? nt[s] : mid;
-    if (f <= k)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  uint32_t r = k;
-  for (int s = 0; s < kSegments; ++s) r -= nt[s] < lo ? nt[s] : lo;
-  uint32_t found = 0;
-  for (int s = 0; s < kSegments; ++s) {
-    if (nt[s] > lo) {
-      if (r == 0) {
-        found = uint32_t(s);
-        break;
-      }
-      --r;
-    }
-  }
-  *seg = found;
-  *round = lo;
-}
 
 // The geometry of one executed pass (the pass-th live digit), written by the
 // scan and read by that pass's scatter kernel in its first instructions.
@@ -140,26 +105,42 @@ struct PassGeo {
   unsigned long long count[kSegments];
 };
 
+// Sticky error bits that kernels OR into a context's device error word; the
+// host reads and clears them (ts_ctx_check, and the calls that wait anyway).
+constexpr uint32_t kErrLookback = 1u;    // a look-back wait hit its bound: the result is wrong
+constexpr uint32_t kErrSegHint = 2u;     // a segment was longer than the caller's max_segment_len
+constexpr uint32_t kErrSegOffsets = 4u;  // segment offsets decreasing or past n
+
 struct SortMeta {
   PassGeo geo[8];
+  // pair[p][s][x]: keys with digit p = x in value segment s of digit p's
+  // pass (the top three bits of digit p-1; for p = 0 the position segment).
   unsigned long long pair[8][kSegments][256];
-  unsigned long long base[8][256];
-  unsigned long long segpre[8][kSegments][256];
-  unsigned long long seg_start[8][kSegments];
-  unsigned long long seg_count[8][kSegments];
-  uint32_t seg_tiles[8][kSegments];
-  uint32_t seg_slot[8][kSegments];
-  uint32_t total_tiles[8];
-  uint32_t tickets[8];
+  // pair_pos[p][s][x]: keys with digit p = x in position segment s, for a
+  // first executed pass on a digit above 0.
+  unsigned long long pair_pos[8][kSegments][256];
+  unsigned long long hist[8][256];  // per digit: all keys
+  unsigned long long base[8][256];  // per digit: exclusive prefix of hist
+  // Per executed pass, segment and digit value: the output position of the
+  // first such key (digit base + the digit's count in earlier segments).
+  unsigned long long pbase[8][kSegments][256];
+  unsigned long long maxb[8];  // per digit: its largest bucket
   uint32_t seg_tickets[8][kSegments];  // per pass and segment: rounds claimed (XCD-affine claims)
-  uint32_t n_live;
+  uint32_t n_live;  // executed passes
   uint32_t live[8];
   uint32_t error;  // non-zero: a look-back wait hit its bound (never expected)
   // Presorted input (presorted.rs:15-65): set by the histogram read when some
   // adjacent pair of radix keys decreases / increases; reverse: the scan
   // found the keys non-increasing and a reverse finishes the sort.
   uint32_t breaks_up, breaks_down, reverse;
-  uint32_t pad[11];
+  // Divert (divert.rs:38-90): skip = live digits left unsorted below the
+  // executed ones (0: none); finish_shift = the lowest sorted bit; fallback:
+  // the finish found a run longer than it sorts that is out of order, and
+  // the gated second sort runs every digit.
+  uint32_t skip, finish_shift, fallback;
+  uint32_t all_live;  // live digits before the divert decision
+  uint32_t diverted;  // the first sort's skip, kept when the second sort runs
+  uint32_t pad[3];
   unsigned long long check[4];  // checksum scratch: sum, xor, unsorted pairs
 };
 
@@ -202,7 +183,7 @@ struct LearnedModel {
 __host__ __device__ __forceinline__ uint32_t log2_fixed(unsigned long long k) {
   if (k == 0) return 0;
   const int lz = TS_CLZ64(k);
-  return uint32_t(63 - lz) << 23 | uint32_t((k << lz) >> 40) & 0x7FFFFFu;
+  return uint32_t(63 - lz) << 23 | (uint32_t((k << lz) >> 40) & 0x7FFFFFu);
 }
 
 __host__ __device__ __forceinline__ unsigned long long learned_x(unsigned long long key, const LearnedModel& m) {

@@ -485,6 +485,8 @@ template <class U> hipError_t run_learned(const LearnedJob& j, hipStream_t s, Ke
   r.keys_b = j.sample_b;
   r.meta = j.meta;
   r.status = j.status;
+  r.status_cap = j.status_cap;
+  r.error = j.error;
   r.num_cus = j.num_cus;
   r.atomic_rank = j.atomic_rank;
   hipError_t e = j.sample > kSmallMax ? radix_sort(r, s, clk)
@@ -518,6 +520,8 @@ template <class U> hipError_t run_learned(const LearnedJob& j, hipStream_t s, Ke
   p.keys_b = j.scratch;
   p.meta = j.meta;
   p.status = j.status;
+  p.status_cap = j.status_cap;
+  p.error = j.error;
   p.num_cus = j.num_cus;
   p.atomic_rank = j.atomic_rank;
   p.model = j.model;

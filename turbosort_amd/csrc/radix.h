@@ -94,35 +94,30 @@ struct SweepShape
   static constexpr int tile = SweepShape::threads * SweepShape::ipt;
 };
 
+// Tile geometry on the host side.  These are defined in radix.hip, the one
+// translation unit that instantiates the scatter kernels, so that workspace
+// sizing and launch shapes always come from the same tile constants.
+//
+// The kind of a sort picks its tile shape: keys only, keys with u32 order
+// values (by-key), or learned bucket digits.
+enum SweepKind : int { kSweepKeys = 0, kSweepVals = 1, kSweepLearned = 2 };
 // Whether a sort of n keys runs the small-tile shape.
-inline bool small_sweep(size_t n, bool learned, bool vals) { return !learned && !vals && n <= TS_SWEEP_SMALL_N; }
-
-inline size_t sweep_tile(int key_bytes, bool learned = false, bool vals = false, bool small = false) {
-  if (small) return size_t(key_bytes < 8 ? SweepShape<4, false, false, true>::tile : SweepShape<8, false, false, true>::tile);
-  if (key_bytes < 8)
-    return size_t(learned ? SweepShape<4, true>::tile : vals ? SweepShape<4, false, true>::tile : SweepShape<4>::tile);
-  return size_t(learned ? SweepShape<8, true>::tile : vals ? SweepShape<8, false, true>::tile : SweepShape<8>::tile);
-}
-// Upper bound on the tiles of one pass: every segment may end in a partial tile.
-inline uint32_t sweep_tiles(size_t n, int key_bytes, bool learned = false, bool vals = false) {
-  const size_t tile = sweep_tile(key_bytes, learned, vals, small_sweep(n, learned, vals));
-  return uint32_t((n + tile - 1) / tile + kSegments);
-}
-// Length of the first pass's position segments (a whole number of tiles).
-inline size_t first_segment(size_t n, int key_bytes, bool learned = false, bool vals = false) {
-  const size_t tile = sweep_tile(key_bytes, learned, vals, small_sweep(n, learned, vals));
-  const size_t per = (n + kSegments - 1) / kSegments;
-  return ((per + tile - 1) / tile) * tile;
-}
-
+bool small_sweep(size_t n, bool learned, bool vals);
+// Keys per tile of the shape a sort of n keys of this kind runs.
+size_t sweep_tile_for(size_t n, int key_bytes, bool learned, bool vals);
+// Upper bound on the tiles of one pass of a sort of n keys: every segment may
+// end in a partial tile.
+uint32_t sweep_tiles(size_t n, int key_bytes, bool learned, bool vals);
 // Look-back words are 32-bit (2 flag bits, 30 value bits) while every count
-// fits in 30 bits, else 64-bit.  Sized for the larger tile count of the two
-// tile shapes (with and without values).
-inline bool wide_status(size_t n) { return n >= (size_t(1) << 30); }
-inline size_t status_bytes(size_t n, int key_bytes, bool learned = false) {
-  const uint32_t tiles = std::max(sweep_tiles(n, key_bytes, learned, false), sweep_tiles(n, key_bytes, learned, true));
-  return size_t(2) * tiles * 256 * (wide_status(n) ? 8 : 4);
-}
+// fits in 30 bits, else 64-bit.
+bool wide_status(size_t n);
+// Bytes of look-back status that one radix sort of n keys of the given kind
+// uses (two alternating rows).
+size_t status_bytes_used(size_t n, int key_bytes, bool learned, bool vals);
+// Bytes of look-back status to reserve so that every radix sort of AT MOST n
+// keys of this width, of any kind, fits: shorter sorts may run the small
+// tiles, which need more rows per key than the large ones.
+size_t status_bytes(size_t n, int key_bytes);
 
 // One LSD sort on the device.  Keys start in keys_a (plain sort) or are read
 // from records (by-key sort; keys_a then receives the transformed keys).
@@ -139,7 +134,15 @@ struct RadixJob {
   uint32_t* final_vals = nullptr;  // by-key: the last pass writes order here
   SortMeta* meta = nullptr;
   void* status = nullptr;
+  size_t status_cap = 0;  // bytes at `status`; a sort needing more is refused
+  // Sticky error bits (device word; kErr* below), kept across sorts until the
+  // host reads and clears them.  Null: meta->error, cleared by each sort.
+  uint32_t* error = nullptr;
   int num_cus = 256;
+  // Keys-only sorts of 8-byte keys: sort only the top live digits that carry
+  // about log2(n) + 2 bits, then finish the runs that share them (the
+  // reference's divert, radix/mod.rs:240-242,268-270, divert.rs:38-90).
+  bool divert = false;
   // Learned partition: the passes sort by the bytes of each key's model
   // bucket (up to 24 bits) instead of by the key (keys only; 4- and 8-byte
   // keys).
@@ -188,6 +191,11 @@ struct SegJob {
   int num_cus = 256;
   uint8_t variant[kSegClasses] = {};  // kernel shape per size class (0: default)
   bool atomic_rank = false;           // as RadixJob::atomic_rank
+  // Sticky error word (as RadixJob::error): the classify ORs in
+  // kErrSegOffsets for invalid offsets and kErrSegHint for a segment longer
+  // than max_len (0: no bound given).
+  uint32_t* error = nullptr;
+  size_t max_len = 0;
   // 0: classify and sort; 1: classify only (counts and lists filled);
   // 2: sort the classes of an earlier stage-1 call.
   int stage = 0;
@@ -262,7 +270,9 @@ struct LearnedJob {
   unsigned long long* offsets = nullptr;  // buckets + 1
   unsigned long long* stats = nullptr;    // [0]: keys in the largest bucket
   SortMeta* meta = nullptr;
-  void* status = nullptr;  // status_bytes(n, key_bytes)
+  void* status = nullptr;  // look-back status for the sample's sort and the partition
+  size_t status_cap = 0;   // bytes at `status`
+  uint32_t* error = nullptr;  // as RadixJob::error
   int num_cus = 256;
   bool atomic_rank = false;  // as RadixJob::atomic_rank
 };
@@ -274,6 +284,9 @@ hipError_t pwl_fit(const float* u, const float* t, uint32_t T, float* theta, flo
                    float lr, hipStream_t s);
 
 hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStream_t s);
-hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, SortMeta* meta, hipStream_t s);
+// Order-independent fingerprint of n keys into out[0..2] (device memory):
+// sum and xor of mix64(bits + salt), and the adjacent pairs out of order.
+hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, unsigned long long* out, hipStream_t s,
+                    KernelClock* clk = nullptr);
 
 }  // namespace ts

@@ -32,9 +32,16 @@ template <class U> __device__ __forceinline__ U first_lane(U x) {
 }
 
 // ------------------------------------------------------------------ init
-__global__ void k_init(SortMeta* m, int P) {
-  for (unsigned i = threadIdx.x; i < unsigned(P) * kSegments * 256; i += blockDim.x) (&m->pair[0][0][0])[i] = 0;
-  if (threadIdx.x < 8) m->tickets[threadIdx.x] = 0;
+// gate: the second sort after a divert (runs only when the finish set
+// m->fallback, and keeps that flag).
+__global__ void k_init(SortMeta* m, int P, int gate) {
+  if (gate && !m->fallback) return;
+  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+  for (unsigned i = g; i < unsigned(P) * kSegments * 256; i += gs) {
+    (&m->pair[0][0][0])[i] = 0;
+    (&m->pair_pos[0][0][0])[i] = 0;
+  }
+  if (blockIdx.x != 0) return;
   if (threadIdx.x < 8 * kSegments) (&m->seg_tickets[0][0])[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     m->n_live = 0;
@@ -42,6 +49,13 @@ __global__ void k_init(SortMeta* m, int P) {
     m->breaks_up = 0;
     m->breaks_down = 0;
     m->reverse = 0;
+    m->skip = 0;
+    m->finish_shift = 0;
+    m->all_live = 0;
+    if (!gate) {
+      m->fallback = 0;
+      m->diverted = 0;
+    }
   }
 }
 
@@ -58,7 +72,10 @@ __global__ void k_init(SortMeta* m, int P) {
 // A digit (with its segment bits) on which all of a wave's keys of one step
 // agree -- constant digits, sorted runs -- is counted with one LDS add
 // instead of 64 colliding atomics.  The grid also clears the first look-back
-// row.
+// row.  Every workgroup's keys lie in one position segment, so its counts
+// summed over value segments also give each digit's position-segment table
+// (pair_pos), which a first executed pass on a digit above 0 uses (divert,
+// or dead low digits).
 #ifndef TS_HIST_THREADS
 #define TS_HIST_THREADS 256
 #endif
@@ -169,11 +186,12 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
                                                        size_t n, int mode, size_t seg0, uint32_t blocks_per_seg,
                                                        size_t per_block, SortMeta* m, uint32_t* zero_ptr,
                                                        size_t zero_words, const LearnedModel* model,
-                                                       int check_order) {
+                                                       int check_order, int gate) {
   using D = typename DigitSource<U, LEARNED>::T;
   constexpr int P = DigitSource<U, LEARNED>::P;
   constexpr int UN = HistUnroll<U>::value;
   constexpr int kHistThreads = HistUnroll<U>::threads;
+  if (gate && !m->fallback) return;
   // Keys already in order (k_order_check; check_order as k_scan's presorted):
   // no pass runs, so no counts.
   if (!LEARNED && check_order && (!m->breaks_up || (check_order == 2 && !m->breaks_down))) return;
@@ -256,69 +274,87 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
     const uint32_t c = h[i];
     if (c) atomicAdd(&m->pair[0][0][0] + i, (unsigned long long)c);
   }
+  // Digit 0's table is already by position segment.
+  for (unsigned i = tid + 256; i < P * 256; i += kHistThreads) {
+    const unsigned p = i >> 8, x = i & 255u;
+    uint32_t c = 0;
+#pragma unroll
+    for (int sg = 0; sg < kSegments; ++sg) c += h[(p * kSegments + sg) * 256 + x];
+    if (c) atomicAdd(&m->pair_pos[p][s0][x], (unsigned long long)c);
+  }
 }
 
 // ------------------------------------------------------------------ scan
+// log2(x) in eighths of a bit, rounded down (the three bits after the leading
+// one stand in for the fraction); x > 0.  The same estimate as the
+// reference's divert decision (divert.rs:95-104).
+__device__ __forceinline__ uint32_t log2_eighths(unsigned long long x) {
+  const uint32_t e = 63u - uint32_t(__clzll((long long)x));
+  const uint32_t frac = e >= 3 ? uint32_t(x >> (e - 3)) & 7u : uint32_t(x << (3 - e)) & 7u;
+  return 8u * e + frac;
+}
+
+// Divert (divert.rs:19-54): walking down from the most significant live
+// digit, each is credited with log2(n / its largest bucket) bits; once the
+// credit reaches log2(n) + 2 bits, the live digits below are left unsorted,
+// provided that skips at least three.
+constexpr uint32_t kDivertSlackBits = 2;
+constexpr uint32_t kDivertMinSkip = 3;
+
 // Per digit p: its histogram (the segments' counts summed), the digit bases
-// (prefix_sum.rs:17-24), each segment's offset inside each digit's run, the
-// segment bounds and tile counts of p's pass, and whether p is live.  A digit
-// is dead when one bin holds every key, as the reference decides from the
-// first key's bin (radix/mod.rs:229-239).  When digit p-1 is dead its value
-// segments collapse into one segment holding all keys in input order, which
-// is the plain single-chain pass, so the segments stay correct whichever
-// digits are live.
+// (prefix_sum.rs:17-24), its largest bucket and whether it is live (one bin
+// holding every key makes it dead, as the reference decides from the first
+// key's bin, radix/mod.rs:229-239).  Then which live digits run (all, or the
+// top ones after a divert), and for each executed pass its segments, their
+// tiles and each (segment, digit value)'s first output position.
+//
+// Segments of executed pass i on digit d: for i = 0 the position segments
+// (the input order), counted in pair_pos (pair for d = 0); for i > 0 the
+// top three bits of digit d-1, counted in pair.  The previous executed digit
+// is d-1, or every digit between them is dead, so that digit d-1 is constant
+// and its value segments collapse into one segment holding all keys in input
+// order -- correct whichever digits are live.
 // presorted: 0 ignores the order check; 1 skips every pass when the keys are
 // already non-decreasing (finish_presorted, presorted.rs:15-27); 2 also
 // reverses non-increasing keys in place instead of sorting them (keys only:
 // equal keys are equal bits, presorted.rs:10-14).
+// divert: 1 lets the scan skip low digits (keys-only sorts that finish the
+// runs with k_finish).  gate: the second sort after a divert, which runs only
+// when the finish set m->fallback.
 __global__ __launch_bounds__(256) void k_scan(SortMeta* m, size_t n, int P, size_t seg0, uint32_t tile,
-                                              int presorted) {
+                                              int presorted, int divert, int gate) {
+  if (gate && !m->fallback) return;
   __shared__ unsigned long long s_wave[4];
-  __shared__ unsigned long long s_prev[256], s_prev_base[256];
+  __shared__ unsigned long long s_hist[8][256], s_base[8][256];
+  __shared__ unsigned long long s_maxw[4];
+  __shared__ unsigned long long s_maxb[8];
   __shared__ uint32_t s_tiles[kSegments];
+  __shared__ uint32_t live[8];
   const unsigned tid = threadIdx.x;
-  uint32_t n_live = 0;
+  uint32_t n_live = 0;  // the same in every lane
   for (int p = 0; p < P; ++p) {
     unsigned long long tot = 0;
 #pragma unroll
-    for (int s = 0; s < kSegments; ++s) {
-      const unsigned long long c = m->pair[p][s][tid];
-      m->segpre[p][s][tid] = tot;
-      tot += c;
-    }
+    for (int s = 0; s < kSegments; ++s) tot += m->pair[p][s][tid];
     const unsigned long long e = block256_exclusive_add(tot, s_wave);
+    s_hist[p][tid] = tot;
+    s_base[p][tid] = e;
+    m->hist[p][tid] = tot;
     m->base[p][tid] = e;
+    unsigned long long mx = tot;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (unsigned long long)__shfl_xor(mx, d, 64));
+    if ((tid & 63) == 0) s_maxw[tid >> 6] = mx;
     const int full = __syncthreads_or(tot == (unsigned long long)n);
+    if (tid == 0) {
+      const unsigned long long b = max(max(s_maxw[0], s_maxw[1]), max(s_maxw[2], s_maxw[3]));
+      s_maxb[p] = b;
+      m->maxb[p] = b;
+    }
     if (!full) {
-      if (tid == 0) m->live[n_live] = uint32_t(p);
+      if (tid == 0) live[n_live] = uint32_t(p);
       ++n_live;
     }
-    if (tid < kSegments) {
-      unsigned long long st, ct;
-      if (p == 0) {
-        st = min((unsigned long long)n, (unsigned long long)tid * seg0);
-        ct = min((unsigned long long)n, st + seg0) - st;
-      } else {
-        ct = 0;
-        for (int x = 0; x < 32; ++x) ct += s_prev[tid * 32 + x];
-        st = s_prev_base[tid * 32];
-      }
-      m->seg_start[p][tid] = st;
-      m->seg_count[p][tid] = ct;
-      s_tiles[tid] = uint32_t((ct + tile - 1) / tile);
-    }
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t slot = 0;
-      for (int s = 0; s < kSegments; ++s) {
-        m->seg_slot[p][s] = slot;
-        m->seg_tiles[p][s] = s_tiles[s];
-        slot += s_tiles[s];
-      }
-      m->total_tiles[p] = slot;
-    }
-    s_prev[tid] = tot;
-    s_prev_base[tid] = e;
   }
   __syncthreads();
   if (presorted >= 1 && m->breaks_up == 0) {
@@ -327,21 +363,66 @@ __global__ __launch_bounds__(256) void k_scan(SortMeta* m, size_t n, int P, size
     n_live = 0;
     if (tid == 0) m->reverse = 1;
   }
-  if (tid == 0) m->n_live = n_live;
+  uint32_t skip = 0;
+  if (divert && n_live > kDivertMinSkip) {
+    const uint32_t log_len = log2_eighths(n), need = log_len + 8 * kDivertSlackBits;
+    uint32_t have = 0;
+    for (int i = int(n_live) - 1; i >= int(kDivertMinSkip); --i) {
+      have += log_len - log2_eighths(s_maxb[live[i]]);
+      if (have >= need) {
+        skip = uint32_t(i);
+        break;
+      }
+    }
+  }
+  const uint32_t n_exec = n_live - skip;
+  if (tid == 0) {
+    m->all_live = n_live;
+    m->n_live = n_exec;
+    m->skip = skip;
+    m->finish_shift = skip ? 8u * live[skip] : 0u;
+    if (!gate) m->diverted = skip;
+    for (uint32_t i = 0; i < n_live; ++i) m->live[i] = live[i];
+  }
   // Geometry of each executed pass, by execution order.
-  for (uint32_t i = 0; i < n_live; ++i) {
-    const uint32_t p = m->live[i];
+  for (uint32_t i = 0; i < n_exec; ++i) {
+    const uint32_t d = live[skip + i];
     PassGeo* g = &m->geo[i];
-    if (tid == 0) {
-      g->digit = p;
-      g->total = m->total_tiles[p];
-    }
     if (tid < kSegments) {
-      g->nt[tid] = m->seg_tiles[p][tid];
-      g->slot[tid] = m->seg_slot[p][tid];
-      g->start[tid] = m->seg_start[p][tid];
-      g->count[tid] = m->seg_count[p][tid];
+      unsigned long long st, ct;
+      if (i == 0) {
+        st = min((unsigned long long)n, (unsigned long long)tid * seg0);
+        ct = min((unsigned long long)n, st + seg0) - st;
+      } else {
+        ct = 0;
+        for (int x = 0; x < 32; ++x) ct += s_hist[d - 1][tid * 32 + x];
+        st = s_base[d - 1][tid * 32];
+      }
+      g->start[tid] = st;
+      g->count[tid] = ct;
+      s_tiles[tid] = uint32_t((ct + tile - 1) / tile);
     }
+    {
+      const unsigned long long(*tab)[256] = (i == 0 && d > 0) ? m->pair_pos[d] : m->pair[d];
+      unsigned long long run = s_base[d][tid];
+#pragma unroll
+      for (int s = 0; s < kSegments; ++s) {
+        m->pbase[i][s][tid] = run;
+        run += tab[s][tid];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t slot = 0;
+      for (int s = 0; s < kSegments; ++s) {
+        g->slot[s] = slot;
+        g->nt[s] = s_tiles[s];
+        slot += s_tiles[s];
+      }
+      g->digit = d;
+      g->total = slot;
+    }
+    __syncthreads();
   }
 }
 
@@ -365,37 +446,17 @@ template <class U> struct SweepArgs {
   int mode_out;  // inverse transform applied by the last pass
   const SortMeta* meta;
   const PassGeo* geo;  // meta->geo, read-only in the scatter
-  uint32_t* tickets;
   uint32_t* seg_tickets;  // [pass][segment]
-  uint32_t* error;  // set when a look-back wait exceeds its bound
+  uint32_t* error;  // kErrLookback is ORed in when a look-back wait exceeds its bound
   uint32_t num_tiles;
   const LearnedModel* model;  // learned partition: digits of the key's bucket
   int atomic_rank;            // rank with LDS atomics (rank_in_wave_atomic), else ballots
+  int gate;                   // the second sort after a divert: run only when meta->fallback is set
 };
-
-#ifndef TS_SWEEP_XCD_AFFINE
-#define TS_SWEEP_XCD_AFFINE 1
-#endif
 
 // About two seconds of s_sleep(1) polling.
 constexpr uint32_t kSpinLimit = 1u << 26;
 
-// One stable scatter pass by the byte digit meta->live[pass] (scatter.rs:26-40).
-//
-// Tiles are claimed through a ticket in an order that interleaves the pass's
-// segments (round t of every segment before round t+1), so every tile a tile
-// waits on has already started, and each segment's look-back chain advances
-// at 1/kSegments of the tile rate.  Each wave ranks its IPT x 64 keys stably
-// (rank_in_wave); the tile's digit counts are published first from an LDS
-// histogram; the exclusive prefix over the segment's earlier tiles is
-// gathered by decoupled look-back, several predecessors per step; and the
-// tile is staged in LDS in digit order so that each digit's keys leave as one
-// coalesced run.  A key's position is digit base + the count of its digit in
-// earlier segments + earlier tiles of its segment + its rank in the tile.
-//
-// Pass 0 of a by-key sort generates the values (positions 0..n-1); the last
-// pass applies the inverse key transform, and in a by-key sort writes only the
-// order to final_vals.
 // LDS of one scatter tile, declared once in the kernel.
 template <class U, bool VALS, int IPT, bool LEARNED = false, bool SMALL = false> struct SweepLds {
   static constexpr int NW = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads / 64;
@@ -570,7 +631,7 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
       j -= used;
       if (used == 0) {
         if (++spins > kSpinLimit) {
-          atomicOr(a.error, 1u);
+          atomicOr(a.error, kErrLookback);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -615,21 +676,25 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
 
 // One stable scatter pass by the byte digit geo[pass].digit (scatter.rs:26-40).
 //
-// Tiles are claimed through a ticket in an order that interleaves the pass's
-// segments (round t of every segment before round t+1), so every tile a tile
-// waits on has already started, and each segment's look-back chain advances
-// at 1/kSegments of the tile rate.  Each wave ranks its IPT x 64 keys stably
-// (rank_in_wave); the tile's digit counts are published first from an LDS
-// histogram; the exclusive prefix over the segment's earlier tiles is
-// gathered by decoupled look-back, several predecessors per step; and the
-// tile is staged in LDS in digit order so that each digit's keys leave as one
-// coalesced run.  A key's position is digit base + the count of its digit in
-// earlier segments + earlier tiles of its segment + its rank in the tile.
+// A workgroup claims a round of the segment numbered like its XCD through a
+// per-(pass, segment) counter, moving on to the following segments once that
+// one is exhausted: a segment's consecutive tiles, whose runs of each digit
+// abut in the output, are then written through one L2.  Rounds of a segment
+// are claimed in order, so a tile's predecessor was claimed earlier by a
+// running workgroup.  Each wave ranks its IPT x 64 keys stably; the tile's
+// digit counts are published; the exclusive prefix over the segment's
+// earlier tiles is gathered by decoupled look-back, several predecessors per
+// step; and the tile is staged in LDS in digit order so that each digit's
+// keys leave as one coalesced run.  A key's position is (the first position
+// of its digit in its segment, pbase) + (earlier tiles of its segment) +
+// (its rank in the tile).
 //
 // Pass 0 of a by-key sort generates the values (positions 0..n-1); the last
 // pass applies the inverse key transform, and in a by-key sort writes only the
-// order to final_vals.
-template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false>
+// order to final_vals.  PERSIST: a grid smaller than the tile count whose
+// workgroups claim tiles until none is left (the gated second sort, which
+// then costs a few microseconds when it does not run).
+template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false, bool PERSIST = false>
 __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads)) void k_sweep(
     SweepArgs<U> a, S* __restrict__ status, int pass) {
   constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::ipt;
@@ -639,12 +704,11 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threa
   __shared__ SweepLds<U, VALS, IPT, LEARNED, SMALL> L;
 
   const SortMeta* __restrict__ meta = a.meta;
+  if (a.gate && !meta->fallback) return;
   const uint32_t n_live = meta->n_live;
   if (uint32_t(pass) >= n_live) return;
   const unsigned tid = threadIdx.x;
-  if (!TS_SWEEP_XCD_AFFINE && tid == 0) L.ticket = atomicAdd(&a.tickets[pass], 1u);
-  // The pass's geometry, from wave-uniform addresses (scalar loads) issued
-  // while the ticket is in flight.
+  // The pass's geometry, from wave-uniform addresses (scalar loads).
   const PassGeo* __restrict__ g = a.geo + pass;
   const unsigned digit = g->digit;
   const uint32_t total = g->total;
@@ -654,24 +718,27 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threa
   const unsigned shift = 8 * digit;
   const bool first = pass == 0;
   const bool last = uint32_t(pass) + 1 == n_live;
+  uint32_t xcc;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
 
-  for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
-  if (tid < 256) L.tcnt[tid] = 0;
   if (LEARNED) load_knots(L.knots, a.model);
-  uint32_t seg, round;
+  // Rows of the next pass's look-back words: a claimed tile clears its
+  // slot's, and the rows past this pass's tiles are cleared here.
   S* next_row = status + size_t((pass + 1) & 1) * a.num_tiles * 256;
-  if (TS_SWEEP_XCD_AFFINE) {
-    // Claim a round of the segment numbered like this workgroup's XCD, so
-    // that a segment's consecutive tiles, whose digit runs abut, write
-    // through one L2; once it is exhausted, of the next segments in turn.
-    // Within a segment rounds are claimed in order, so a tile's predecessor
-    // was claimed first by a running workgroup.
+  if (tid < 256) {
+    if (PERSIST) {
+      for (uint32_t b = total + blockIdx.x; b < a.num_tiles; b += gridDim.x) next_row[size_t(b) * 256 + tid] = 0;
+    } else if (blockIdx.x >= total && blockIdx.x < a.num_tiles) {
+      next_row[size_t(blockIdx.x) * 256 + tid] = 0;
+    }
+  }
+  while (true) {
+    for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
+    if (tid < 256) L.tcnt[tid] = 0;
     if (tid == 0) {
-      uint32_t x;
-      asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
       uint32_t got = kSegments, r = 0;
       for (uint32_t i = 0; i < uint32_t(kSegments); ++i) {
-        const uint32_t sg = (x + i) & uint32_t(kSegments - 1);
+        const uint32_t sg = (xcc + i) & uint32_t(kSegments - 1);
         if (nt[sg] == 0) continue;
         r = atomicAdd(&a.seg_tickets[pass * kSegments + sg], 1u);
         if (r < nt[sg]) {
@@ -683,38 +750,29 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threa
     }
     __syncthreads();
     const uint32_t claim = __builtin_amdgcn_readfirstlane(L.ticket);
-    // Rows of the next pass's look-back words: a claimed tile clears its
-    // slot's, and workgroup b clears row b when b is no tile's slot.
-    if (tid < 256 && blockIdx.x >= total && blockIdx.x < a.num_tiles) next_row[size_t(blockIdx.x) * 256 + tid] = 0;
-    seg = claim >> 24;
+    const uint32_t seg = claim >> 24;
     if (seg >= uint32_t(kSegments)) return;
-    round = claim & 0xFFFFFFu;
+    const uint32_t round = claim & 0xFFFFFFu;
     if (tid < 256) next_row[(size_t(g->slot[seg]) + round) * 256 + tid] = 0;
-  } else {
-    __syncthreads();
-    const uint32_t ticket = __builtin_amdgcn_readfirstlane(L.ticket);
-    if (tid < 256) next_row[size_t(ticket) * 256 + tid] = 0;  // the next pass's look-back row
-    if (ticket >= total) return;
-    ticket_to_tile(ticket, nt, &seg, &round);  // uniform: stays in scalar registers
-    seg = __builtin_amdgcn_readfirstlane(seg);
-    round = __builtin_amdgcn_readfirstlane(round);
-  }
 
-  const size_t seg_lo = g->start[seg];
-  const size_t seg_hi = seg_lo + g->count[seg];
-  const size_t tile_base = seg_lo + size_t(round) * TILE;
-  const uint32_t valid = seg_hi - tile_base >= size_t(TILE) ? uint32_t(TILE) : uint32_t(seg_hi - tile_base);
-  const size_t first_slot = g->slot[seg];
-  const size_t slot = first_slot + round;
-  // This lane's digit: its base and its count in earlier segments.
-  const unsigned long long digit_base =
-      tid < 256 ? meta->base[digit][tid] + meta->segpre[digit][seg][tid] : 0ull;
-  if (valid == uint32_t(TILE))
-    run_tile<U, VALS, S, true, LEARNED, SMALL>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
-                                                 slot, round == 0, digit_base);
-  else
-    run_tile<U, VALS, S, false, LEARNED, SMALL>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
-                                                  slot, round == 0, digit_base);
+    const size_t seg_lo = g->start[seg];
+    const size_t seg_hi = seg_lo + g->count[seg];
+    const size_t tile_base = seg_lo + size_t(round) * TILE;
+    const uint32_t valid = seg_hi - tile_base >= size_t(TILE) ? uint32_t(TILE) : uint32_t(seg_hi - tile_base);
+    const size_t first_slot = g->slot[seg];
+    const size_t slot = first_slot + round;
+    // This lane's digit: the output position of the first key of that digit
+    // in this segment.
+    const unsigned long long digit_base = tid < 256 ? meta->pbase[pass][seg][tid] : 0ull;
+    if (valid == uint32_t(TILE))
+      run_tile<U, VALS, S, true, LEARNED, SMALL>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
+                                                   slot, round == 0, digit_base);
+    else
+      run_tile<U, VALS, S, false, LEARNED, SMALL>(L, a, status, pass, shift, first, last, tile_base, valid,
+                                                    first_slot, slot, round == 0, digit_base);
+    if (!PERSIST) return;
+    __syncthreads();  // the tile's LDS is reused by the next claim
+  }
 }
 
 // Non-increasing keys found by the scan: reversing them sorts them.  Each
@@ -745,11 +803,11 @@ template <class U> __device__ __forceinline__ void reverse_keys(U* __restrict__ 
 
 // The last step of a keys-only sort: an odd number of live passes leaves the
 // keys in the scratch half, and keys the scan found non-increasing are
-// reversed.
+// reversed.  After a divert the finish has already written the output.
 template <class U>
 __global__ void k_copy_back(const SortMeta* m, const U* __restrict__ src, U* __restrict__ dst, size_t n) {
   if (m->reverse) return reverse_keys(dst, n);
-  if ((m->n_live & 1u) == 0) return;
+  if (m->skip || (m->n_live & 1u) == 0) return;
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
   for (size_t i = g; i < n; i += gs) dst[i] = src[i];
 }
@@ -762,7 +820,139 @@ __global__ void k_iota_if_dead(const SortMeta* m, uint32_t* out, size_t n) {
   for (size_t i = g; i < n; i += gs) out[i] = uint32_t(i);
 }
 
-template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk) {
+// ---------------------------------------------------------------- finish
+// After a divert the keys are in order of their bits from finish_shift up
+// (the prefix), and each run of keys sharing a prefix only needs sorting
+// within itself (finish_runs, divert.rs:60-90).  A workgroup takes a tile of
+// kFinishTile positions and loads them with kFinishHalo keys on either side
+// into LDS.  Every run of at most kFinishHalo keys is sorted by the tile that
+// holds its first key: each key's rank is the number of keys of its run that
+// are smaller, or equal and earlier (the order among equal keys is not
+// observable).  A longer run is left as it is when it is already in order (a
+// run of one repeated key, say); otherwise the finish sets m->fallback and
+// the gated second sort sorts every digit.  Every tile classifies every run
+// the same way (short runs lie inside the window of each tile they touch).
+//
+// The data is in the first buffer after an even number of executed passes:
+// the finish then works in place and writes only keys that move; a tile
+// never writes a run it does not own, and what it reads of other runs is
+// only their prefixes, which moves inside a run do not change.  After an odd
+// number it moves every key from the second buffer into the first.
+constexpr int kFinishThreads = 256;
+constexpr int kFinishTile = kFinishThreads * 16;
+constexpr int kFinishHalo = 256;
+constexpr int kFinishWin = kFinishTile + 2 * kFinishHalo;
+constexpr int kFinishChunk = kFinishWin / kFinishThreads;  // window positions per lane
+static_assert(kFinishWin % kFinishThreads == 0, "whole chunks");
+static_assert(kFinishHalo < kFinishTile, "a short run touches at most two tiles");
+
+// Exclusive scan of one int per lane over the workgroup: the max over the
+// earlier lanes (FWD) or the min over the later lanes, `id` when there are
+// none.  s holds one slot per wave.
+template <bool FWD> __device__ __forceinline__ int block_excl_extreme(int v, int id, int* s) {
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr unsigned NW = kFinishThreads / 64;
+  int incl = v;
+#pragma unroll
+  for (unsigned d = 1; d < 64; d <<= 1) {
+    const int o = FWD ? __shfl_up(incl, d, 64) : __shfl_down(incl, d, 64);
+    if (FWD ? lane >= d : lane + d < 64u) incl = FWD ? max(incl, o) : min(incl, o);
+  }
+  int ex = FWD ? __shfl_up(incl, 1u, 64) : __shfl_down(incl, 1u, 64);
+  if (lane == (FWD ? 0u : 63u)) ex = id;
+  if (lane == (FWD ? 63u : 0u)) s[w] = incl;
+  __syncthreads();
+#pragma unroll
+  for (unsigned x = 0; x < NW; ++x)
+    if (FWD ? x < w : x > w) ex = FWD ? max(ex, s[x]) : min(ex, s[x]);
+  __syncthreads();
+  return ex;
+}
+
+template <class U>
+__global__ __launch_bounds__(kFinishThreads) void k_finish(SortMeta* m, U* __restrict__ keys_a,
+                                                           const U* __restrict__ keys_b, size_t n, int mode) {
+  if (m->skip == 0) return;
+  const uint32_t shift = m->finish_shift;
+  const bool in_place = (m->n_live & 1u) == 0;
+  const U* src = in_place ? keys_a : keys_b;
+  U* dst = keys_a;
+  __shared__ U w[kFinishWin];
+  __shared__ int s_scan[kFinishThreads / 64];
+  const unsigned tid = threadIdx.x;
+  constexpr int kNone = 1 << 30;
+  auto pre = [&](U k) { return uint64_t(k) >> shift; };
+  bool unsorted = false;
+  for (size_t tile = blockIdx.x; tile * kFinishTile < n; tile += gridDim.x) {
+    const size_t base = tile * kFinishTile;
+    const size_t lo = base >= size_t(kFinishHalo) ? base - kFinishHalo : 0;
+    const size_t hi = min(n, base + kFinishTile + kFinishHalo);
+    const int wn = int(hi - lo);
+    const int own0 = int(base - lo), own1 = int(min(n, base + kFinishTile) - lo);  // this tile: [own0, own1)
+    __syncthreads();  // the previous tile's window is no longer read
+    for (int q = int(tid); q < wn; q += kFinishThreads) w[q] = to_key(src[lo + size_t(q)], mode);
+    __syncthreads();
+    // Run starts in this lane's chunk of the window: position q starts a run
+    // when its prefix differs from q-1's (window position 0 only at the
+    // array's start; before the window nothing is known).
+    const int c0 = int(tid) * kFinishChunk;
+    uint32_t bmask = 0;
+#pragma unroll
+    for (int r = 0; r < kFinishChunk; ++r) {
+      const int q = c0 + r;
+      if (q < wn && (q == 0 ? lo == 0 : pre(w[q]) != pre(w[q - 1]))) bmask |= 1u << r;
+    }
+    const int chunk_last = bmask ? c0 + 31 - __clz(bmask) : -1;
+    const int chunk_first = bmask ? c0 + __ffs(bmask) - 1 : kNone;
+    const int start_before = block_excl_extreme<true>(chunk_last, -1, s_scan);
+    const int start_after = block_excl_extreme<false>(chunk_first, kNone, s_scan);
+    // A run with no start after it inside the window ends at the window's
+    // end if that is the array's end, else somewhere unknown.
+    const int end_default = min(start_after, hi == n ? wn : kNone);
+#pragma unroll 1
+    for (int r = 0; r < kFinishChunk; ++r) {
+      const int q = c0 + r;
+      if (q < own0 || q >= wn) continue;
+      const uint32_t upto = bmask & ((2u << r) - 1u), later = bmask & ~((2u << r) - 1u);
+      const int rs = upto ? c0 + 31 - __clz(upto) : start_before;  // run start (-1: before the window)
+      const int re = later ? c0 + __ffs(later) - 1 : end_default;  // run end (kNone: past the window)
+      const bool short_run = rs >= 0 && re != kNone && re - rs <= kFinishHalo;
+      const U k = w[q];
+      if (short_run) {
+        if (rs < own0 || rs >= own1) continue;  // another tile's run
+        if (re - rs == 1) {
+          if (!in_place) dst[lo + size_t(q)] = from_key(k, mode);
+          continue;
+        }
+        int rank = 0;
+        for (int j = rs; j < re; ++j) {
+          const U x = w[j];
+          rank += (x < k) || (x == k && j < q);
+        }
+        const int o = rs + rank;
+        if (!in_place || o != q) dst[lo + size_t(o)] = from_key(k, mode);
+      } else {
+        if (q >= own1) continue;  // the next tile's position
+        if (!in_place) dst[lo + size_t(q)] = from_key(k, mode);
+        // q is not the run's start, so q - 1 lies in the same run.
+        if (rs != q && w[q - 1] > k) unsorted = true;
+      }
+    }
+  }
+  if (__syncthreads_or(unsorted) && tid == 0) atomicOr(&m->fallback, 1u);
+}
+
+// Length of the first pass's position segments (a whole number of tiles).
+size_t first_segment(size_t n, int key_bytes, bool learned, bool vals) {
+  const size_t tile = sweep_tile_for(n, key_bytes, learned, vals);
+  const size_t per = (n + kSegments - 1) / kSegments;
+  return ((per + tile - 1) / tile) * tile;
+}
+
+// gate: the second sort after a divert (persistent launches for keys-only
+// 8-byte keys, run only when the finish found an out-of-order long run).
+template <class U, class S, bool LEARNED>
+hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool gate) {
   const bool vals = j.final_vals != nullptr;
   const uint32_t tiles = sweep_tiles(j.n, sizeof(U), LEARNED, vals);
   SweepArgs<U> a;
@@ -776,26 +966,36 @@ template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& 
   a.mode_out = a.mode_in;
   a.meta = j.meta;
   a.geo = j.meta->geo;
-  a.tickets = j.meta->tickets;
   a.seg_tickets = &j.meta->seg_tickets[0][0];
-  a.error = &j.meta->error;
+  a.error = j.error ? j.error : &j.meta->error;
   a.num_tiles = tiles;
   a.model = j.model;
   a.atomic_rank = j.atomic_rank ? 1 : 0;
+  a.gate = gate ? 1 : 0;
   const double bytes = double(j.n) * (2.0 * sizeof(U) + (vals ? 8.0 : 0.0));
   for (int p = 0; p < DigitSource<U, LEARNED>::P; ++p) {
-    if (clk) clk->begin(s, "k_sweep", bytes);
-    if constexpr (LEARNED)
+    if (clk) clk->begin(s, gate ? "k_sweep_fallback" : "k_sweep", bytes);
+    if constexpr (LEARNED) {
       hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U), true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
-    else if (vals)
+    } else if (vals) {
       hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U), false, true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
-    else if (small_sweep(j.n, false, false))
+    } else if (gate) {
+      if constexpr (sizeof(U) == 8) {
+        const unsigned grid = unsigned(std::min<size_t>(tiles, size_t(j.num_cus)));
+        hipLaunchKernelGGL((k_sweep<U, false, S, false, false, true>), dim3(grid), dim3(SweepShape<sizeof(U)>::threads),
+                           0, s, a, static_cast<S*>(j.status), p);
+      } else {
+        return hipErrorInvalidValue;
+      }
+    } else if (small_sweep(j.n, false, false)) {
       hipLaunchKernelGGL((k_sweep<U, false, S, false, true>), dim3(tiles),
                          dim3(SweepShape<sizeof(U), false, false, true>::threads), 0, s, a, static_cast<S*>(j.status), p);
-    else
-      hipLaunchKernelGGL((k_sweep<U, false, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a, static_cast<S*>(j.status), p);
+    } else {
+      hipLaunchKernelGGL((k_sweep<U, false, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a,
+                         static_cast<S*>(j.status), p);
+    }
     if (clk) clk->end(s);
   }
   return hipGetLastError();
@@ -804,10 +1004,14 @@ template <class U, class S, bool LEARNED> hipError_t run_sweeps(const RadixJob& 
 template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j, hipStream_t s, KernelClock* clk) {
   constexpr int P = DigitSource<U, LEARNED>::P;
   constexpr int KB = int(sizeof(U));
-  hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, s, j.meta, P);
+  const bool vals = j.final_vals != nullptr;
+  // The look-back rows this sort's launches index must lie inside the
+  // caller's status area.
+  if (!j.status || status_bytes_used(j.n, KB, LEARNED, vals) > j.status_cap) return hipErrorInvalidValue;
+  const bool divert = !LEARNED && !vals && !j.records && KB == 8 && j.divert && !small_sweep(j.n, false, false);
+  hipLaunchKernelGGL(k_init, dim3(16), dim3(256), 0, s, j.meta, P, 0);
   // Histogram: blocks_per_seg workgroups per position segment, each over a
   // contiguous chunk of it.
-  const bool vals = j.final_vals != nullptr;
   const size_t seg0 = first_segment(j.n, KB, LEARNED, vals);
   // Histogram workgroups per CU.  The learned histogram holds three digit
   // tables (and the knots), not eight, so more fit a CU;
@@ -822,6 +1026,7 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   const uint32_t tiles = sweep_tiles(j.n, KB, LEARNED, vals);
   const size_t zero_words = size_t(tiles) * 256 * (wide_status(j.n) ? 2 : 1);
   const dim3 hgrid(blocks_per_seg * kSegments);
+  const uint32_t tile = uint32_t(sweep_tile_for(j.n, KB, LEARNED, vals));
   // The order check does not apply to bucket digits, and only keys without
   // values may be reversed (a stable order of non-increasing keys with ties
   // is no reversal).
@@ -836,26 +1041,44 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
       hipLaunchKernelGGL((k_order_check<U, false>), dim3(og), dim3(256), 0, s, static_cast<const U*>(j.keys_a),
                          nullptr, size_t(0), size_t(0), j.n, j.mode, j.meta);
   }
-  if (clk) clk->begin(s, "k_hist", double(j.n) * (j.records ? double(j.stride) + KB : KB));
-  if constexpr (LEARNED)
-    hipLaunchKernelGGL((k_hist<U, false, true>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
-                       nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                       static_cast<uint32_t*>(j.status), zero_words, j.model, check_order);
-  else if (j.records)
-    hipLaunchKernelGGL((k_hist<U, true>), hgrid, dim3(kHistThreads), 0, s, nullptr, j.records, j.stride,
-                       j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                       static_cast<uint32_t*>(j.status), zero_words, nullptr, check_order);
-  else
-    hipLaunchKernelGGL((k_hist<U, false>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a), nullptr,
-                       size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                       static_cast<uint32_t*>(j.status), zero_words, nullptr, check_order);
-  if (clk) clk->end(s);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0,
-                     uint32_t(sweep_tile(KB, LEARNED, vals, small_sweep(j.n, LEARNED, vals))),
-                     check_order);
-  hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk)
-                                  : run_sweeps<U, uint32_t, LEARNED>(j, s, clk);
+  auto hist = [&](int gate) {
+    const int co = gate ? 0 : check_order;
+    if (clk) clk->begin(s, gate ? "k_hist_fallback" : "k_hist", double(j.n) * (j.records ? double(j.stride) + KB : KB));
+    if constexpr (LEARNED)
+      hipLaunchKernelGGL((k_hist<U, false, true>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
+                         nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
+                         static_cast<uint32_t*>(j.status), zero_words, j.model, co, gate);
+    else if (j.records)
+      hipLaunchKernelGGL((k_hist<U, true>), hgrid, dim3(kHistThreads), 0, s, nullptr, j.records, j.stride,
+                         j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
+                         static_cast<uint32_t*>(j.status), zero_words, nullptr, co, gate);
+    else
+      hipLaunchKernelGGL((k_hist<U, false>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
+                         nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
+                         static_cast<uint32_t*>(j.status), zero_words, nullptr, co, gate);
+    if (clk) clk->end(s);
+  };
+  hist(0);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, tile, check_order, divert ? 1 : 0, 0);
+  hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk, false)
+                                  : run_sweeps<U, uint32_t, LEARNED>(j, s, clk, false);
   if (e != hipSuccess) return e;
+  if (divert) {
+    if constexpr (!LEARNED && KB == 8) {
+      // Finish the runs, then the second sort that runs only if a long run
+      // was out of order.
+      if (clk) clk->begin(s, "k_finish", double(j.n) * KB);
+      hipLaunchKernelGGL((k_finish<U>), dim3(unsigned(j.num_cus) * 4), dim3(kFinishThreads), 0, s, j.meta,
+                         static_cast<U*>(j.keys_a), static_cast<const U*>(j.keys_b), j.n, j.mode);
+      if (clk) clk->end(s);
+      hipLaunchKernelGGL(k_init, dim3(16), dim3(256), 0, s, j.meta, P, 1);
+      hist(1);
+      hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, tile, 0, 0, 1);
+      e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk, true)
+                           : run_sweeps<U, uint32_t, LEARNED>(j, s, clk, true);
+      if (e != hipSuccess) return e;
+    }
+  }
   const unsigned cb = unsigned(std::min<size_t>((j.n + 255) / 256, size_t(j.num_cus) * 8));
   if (j.final_vals) {
     hipLaunchKernelGGL(k_iota_if_dead, dim3(cb ? cb : 1), dim3(256), 0, s, j.meta, j.final_vals, j.n);
@@ -912,6 +1135,44 @@ __global__ __launch_bounds__(512) void k_rank_check(uint32_t* bad) {
 }
 
 }  // namespace
+
+// ---------------------------------------------------------- tile geometry
+namespace {
+size_t tile_of(int key_bytes, bool learned, bool vals, bool small) {
+  if (small) return size_t(key_bytes < 8 ? SweepShape<4, false, false, true>::tile : SweepShape<8, false, false, true>::tile);
+  if (key_bytes < 8)
+    return size_t(learned ? SweepShape<4, true>::tile : vals ? SweepShape<4, false, true>::tile : SweepShape<4>::tile);
+  return size_t(learned ? SweepShape<8, true>::tile : vals ? SweepShape<8, false, true>::tile : SweepShape<8>::tile);
+}
+size_t status_row_bytes(size_t n) { return size_t(256) * (wide_status(n) ? 8 : 4); }
+}  // namespace
+
+bool small_sweep(size_t n, bool learned, bool vals) { return !learned && !vals && n <= TS_SWEEP_SMALL_N; }
+
+size_t sweep_tile_for(size_t n, int key_bytes, bool learned, bool vals) {
+  return tile_of(key_bytes, learned, vals, small_sweep(n, learned, vals));
+}
+
+uint32_t sweep_tiles(size_t n, int key_bytes, bool learned, bool vals) {
+  const size_t tile = sweep_tile_for(n, key_bytes, learned, vals);
+  return uint32_t((n + tile - 1) / tile + kSegments);
+}
+
+bool wide_status(size_t n) { return n >= (size_t(1) << 30); }
+
+size_t status_bytes_used(size_t n, int key_bytes, bool learned, bool vals) {
+  return size_t(2) * sweep_tiles(n, key_bytes, learned, vals) * status_row_bytes(n);
+}
+
+size_t status_bytes(size_t n, int key_bytes) {
+  // Per kind the row count grows with n, except that keys-only sorts of at
+  // most TS_SWEEP_SMALL_N keys switch to smaller tiles: the longest such sort
+  // bounds them.  Rows are as wide as n's.
+  size_t rows = std::max({sweep_tiles(n, key_bytes, false, false), sweep_tiles(n, key_bytes, false, true),
+                          sweep_tiles(n, key_bytes, true, false),
+                          sweep_tiles(std::min<size_t>(n, TS_SWEEP_SMALL_N), key_bytes, false, false)});
+  return size_t(2) * rows * status_row_bytes(n);
+}
 
 hipError_t rank_order_check(uint32_t* bad, hipStream_t s) {
   hipLaunchKernelGGL(k_rank_check, dim3(64), dim3(512), 0, s, bad);

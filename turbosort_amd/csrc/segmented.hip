@@ -42,6 +42,8 @@ struct SegArgs {
   uint32_t caps[kSegClasses];
   int mode;
   int atomic_rank;  // byte-digit ranks by LDS atomic adds (rank_in_wave_atomic), else ballots
+  uint32_t* error;  // sticky error word, or null
+  unsigned long long max_len;  // the caller's bound on segment lengths (0: none)
 };
 
 __device__ __forceinline__ unsigned long long seg_end(const SegArgs& a, size_t s) {
@@ -87,8 +89,10 @@ __global__ __launch_bounds__(kClassifyThreads) void k_seg_classify(SegArgs a) {
       const unsigned long long lo = a.offsets[s], hi = seg_end(a, s);
       if (hi < lo || hi > a.n) {
         a.counts[kSegClasses] = 1;
+        if (a.error) atomicOr(a.error, kErrSegOffsets);
       } else if (hi - lo >= 2) {
         const unsigned long long len = hi - lo;
+        if (a.max_len && len > a.max_len && a.error) atomicOr(a.error, kErrSegHint);
         c = kSegDevice;
         for (int x = 0; x < kSegDevice; ++x) {
           if (len <= a.caps[x]) {
@@ -1111,6 +1115,8 @@ template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStrea
   a.lists = j.lists;
   a.mode = j.mode;
   a.atomic_rank = j.atomic_rank ? 1 : 0;
+  a.error = j.error;
+  a.max_len = j.max_len;
   for (int c = 0; c < kSegClasses; ++c) a.caps[c] = c < kSegDevice ? class_cap<U, VALS>(c) : 0u;
   if (j.stage != 2) {
     hipLaunchKernelGGL(k_seg_clear, dim3(1), dim3(64), 0, s, j.counts);

@@ -58,14 +58,14 @@ __global__ void k_splitmix(void* out, size_t n, uint64_t seed, int kind) {
   }
 }
 
-__global__ void k_check_zero(SortMeta* m) {
-  if (threadIdx.x < 4) m->check[threadIdx.x] = 0;
+__global__ void k_check_zero(unsigned long long* out) {
+  if (threadIdx.x < 3) out[threadIdx.x] = 0;
 }
 
 // Sum and xor of mix64(bits) over the keys (a multiset fingerprint that any
 // permutation preserves) and the number of adjacent pairs out of order.
 template <class U>
-__global__ void k_check(const U* __restrict__ x, size_t n, int mode, SortMeta* m) {
+__global__ void k_check(const U* __restrict__ x, size_t n, int mode, unsigned long long* out) {
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
   unsigned long long sum = 0, xr = 0, bad = 0;
   for (size_t i = g; i < n; i += gs) {
@@ -96,9 +96,9 @@ __global__ void k_check(const U* __restrict__ x, size_t n, int mode, SortMeta* m
       xr ^= part[1][i];
       bad += part[2][i];
     }
-    atomicAdd(&m->check[0], sum);
-    atomicXor(&m->check[1], xr);
-    atomicAdd(&m->check[2], bad);
+    atomicAdd(&out[0], sum);
+    atomicXor(&out[1], xr);
+    atomicAdd(&out[2], bad);
   }
 }
 
@@ -141,29 +141,32 @@ hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStre
   return hipGetLastError();
 }
 
-hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, SortMeta* meta, hipStream_t s) {
-  hipLaunchKernelGGL(k_check_zero, dim3(1), dim3(64), 0, s, meta);
+hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, unsigned long long* out, hipStream_t s,
+                    KernelClock* clk) {
+  hipLaunchKernelGGL(k_check_zero, dim3(1), dim3(64), 0, s, out);
   if (n == 0) return hipGetLastError();
   const unsigned grid = std::min(grid_for(n, 8), 2048u);
+  if (clk) clk->begin(s, "k_check", double(n) * key_bytes);
   switch (key_bytes) {
     case 1:
       hipLaunchKernelGGL(k_check<uint8_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint8_t*>(data), n,
-                         mode, meta);
+                         mode, out);
       break;
     case 2:
       hipLaunchKernelGGL(k_check<uint16_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint16_t*>(data), n,
-                         mode, meta);
+                         mode, out);
       break;
     case 4:
       hipLaunchKernelGGL(k_check<uint32_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint32_t*>(data), n,
-                         mode, meta);
+                         mode, out);
       break;
     case 8:
       hipLaunchKernelGGL(k_check<uint64_t>, dim3(grid), dim3(kThreads), 0, s, static_cast<const uint64_t*>(data), n,
-                         mode, meta);
+                         mode, out);
       break;
     default: return hipErrorInvalidValue;
   }
+  if (clk) clk->end(s);
   return hipGetLastError();
 }
 
