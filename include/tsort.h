@@ -66,6 +66,19 @@ ts_status ts_sort(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, void* str
 ts_status ts_sort_with_buffer(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, void* d_buf,
                               size_t buf_len, void* stream);
 
+/* No counterpart in the reference: ts_sort_with_buffer for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, checks, messages
+ * and output as ts_sort_with_buffer; n is at most 2^28 keys (larger n is
+ * TS_ERR_INVALID_ARG).  Nothing is allocated per call: the sort's
+ * bookkeeping and look-back status live in one file-scope device array of
+ * about 30 MB that each GPU the library runs on holds until the process
+ * ends (its address is looked up in ts_ctx_create).  Every call on one GPU
+ * shares that array, whatever the context: calls that overlap on one GPU
+ * race and give wrong results; serialize them (one stream, or events). */
+ts_status ts_sort_with_buffer_static(ts_ctx* ctx, ts_dtype dtype, void* d_data, size_t n, void* d_buf,
+                                     size_t buf_len, void* stream);
+
 /* turbosort::sort (lib.rs:99-101) on a slice in host memory: copies it to the device
  * (in 64 MiB pieces; a pinned slice streams at the link rate, a pageable one
  * through the runtime's staging), sorts it there and copies it back, and

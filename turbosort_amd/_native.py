@@ -22,7 +22,7 @@ EXPORTS = [
     "ts_segmented_variants", "ts_segmented_variant_name", "ts_segmented_set_variant",
     "ts_ctx_set_ranking", "ts_ctx_ranking", "ts_sort_host", "ts_ctx_set_split_rounds", "ts_ctx_split_rounds",
     "ts_ctx_set_divert_min", "ts_ctx_divert_min", "ts_ctx_check", "ts_ctx_last_sort_info", "ts_plan_status_bytes",
-    "ts_verify_sorted",
+    "ts_verify_sorted", "ts_sort_with_buffer_static",
 ]
 
 
@@ -134,6 +134,8 @@ def load() -> ctypes.CDLL:
     L.ts_plan_status_bytes.restype = sz
     L.ts_verify_sorted.argtypes = [vp, i32, vp, sz, ctypes.POINTER(u64), vp]
     L.ts_verify_sorted.restype = i32
+    L.ts_sort_with_buffer_static.argtypes = [vp, i32, vp, sz, vp, sz, vp]
+    L.ts_sort_with_buffer_static.restype = i32
     L.ts_ctx_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_double), i32]
     L.ts_ctx_kernel_times.restype = i32

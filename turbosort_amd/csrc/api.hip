@@ -52,6 +52,9 @@ struct ts_ctx {
   uint32_t* dflags = nullptr;
   // Sorts of 8-byte keys without values divert from this many keys on.
   size_t divert_min = size_t(1) << 23;
+  // Bookkeeping of the last radix sort (the workspace, or the static
+  // scratch), for ts_ctx_last_sort_info.
+  const ts::SortMeta* last_meta = nullptr;
 };
 
 namespace {
@@ -90,6 +93,9 @@ ts_status ensure(ts_ctx* c, size_t bytes) {
   if (bytes <= c->ws_bytes) return TS_OK;
   if (c->ws) {
     hipError_t e = hipFree(c->ws);  // waits for work still using the old workspace
+    if (c->last_meta && reinterpret_cast<const char*>(c->last_meta) >= c->ws &&
+        reinterpret_cast<const char*>(c->last_meta) < c->ws + c->ws_bytes)
+      c->last_meta = nullptr;
     c->ws = nullptr;
     c->ws_bytes = 0;
     if (e != hipSuccess) return hip_fail(c, e, "hipFree");
@@ -166,7 +172,8 @@ ts_status check_flags(ts_ctx* c, hipStream_t s) {
 }
 
 // A radix job on the context's device with its sticky error word.
-ts::RadixJob radix_job(const ts_ctx* c, int kb, int mode, size_t n, char* meta, char* status, size_t status_cap) {
+ts::RadixJob radix_job(ts_ctx* c, int kb, int mode, size_t n, char* meta, char* status, size_t status_cap) {
+  c->last_meta = reinterpret_cast<const ts::SortMeta*>(meta);
   ts::RadixJob j;
   j.key_bytes = kb;
   j.mode = mode;
@@ -195,6 +202,48 @@ size_t by_key_ws(size_t n, int kb, size_t gather_bytes) {
 }
 
 void clock_reset(ts_ctx* c) { c->clk.count = 0; }
+
+// ---- Sorting without allocation (ts_sort_with_buffer_static) ----
+// The bookkeeping and look-back status of keys-only radix sorts of up to
+// kStaticMaxKeys keys live in `scratch`, a file-scope device array of which
+// each GPU holds its own copy for the life of the process.  Host code uses
+// the array only through the per-device address hipGetSymbolAddress gives
+// (static_scratch_setup, at context creation); the array's name on the host
+// is not a device address.
+constexpr size_t kStaticMaxKeys = size_t(1) << 28;
+constexpr size_t static_tiles(size_t n, size_t tile) { return (n + tile - 1) / tile + size_t(ts::kSegments); }
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+// Keys-only shapes: the large tiles up to kStaticMaxKeys keys, the small
+// tiles up to TS_SWEEP_SMALL_N (keys of 1, 2 and 4 bytes share the 4-byte
+// shapes).
+constexpr size_t kStaticTiles =
+    cmax(cmax(static_tiles(kStaticMaxKeys, ts::SweepShape<4>::tile), static_tiles(kStaticMaxKeys, ts::SweepShape<8>::tile)),
+         cmax(static_tiles(TS_SWEEP_SMALL_N, ts::SweepShape<4, false, false, true>::tile),
+              static_tiles(TS_SWEEP_SMALL_N, ts::SweepShape<8, false, false, true>::tile)));
+static_assert(kStaticMaxKeys < (size_t(1) << 30), "32-bit look-back words");
+constexpr size_t kStaticMeta = (sizeof(ts::SortMeta) + 255) / 256 * 256;
+constexpr size_t kStaticStatus = 2 * kStaticTiles * 256 * 4;
+__device__ __align__(256) static unsigned char scratch[kStaticMeta + kStaticStatus];
+
+std::mutex g_static_mu;
+std::vector<void*> g_static_ptr;  // per device: the address of `scratch` there
+
+// With `device` current: looks up the device's copy of `scratch` once.
+hipError_t static_scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (size_t(device) < g_static_ptr.size() && g_static_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (size_t(device) >= g_static_ptr.size()) g_static_ptr.resize(size_t(device) + 1, nullptr);
+  g_static_ptr[device] = p;
+  return hipSuccess;
+}
+
+char* static_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  return size_t(device) < g_static_ptr.size() ? static_cast<char*>(g_static_ptr[device]) : nullptr;
+}
 
 __global__ void k_iota(uint32_t* out, size_t n) {
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
@@ -525,6 +574,7 @@ ts_status ts_ctx_create(ts_ctx** out, int device) {
     DeviceScope ds(device);
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&c->dflags), 256);
     if (e == hipSuccess) e = hipMemset(c->dflags, 0, 256);
+    if (e == hipSuccess) e = static_scratch_setup(device);
     if (e != hipSuccess) {
       if (c->dflags) (void)hipFree(c->dflags);
       delete c;
@@ -582,10 +632,10 @@ ts_status ts_ctx_check(ts_ctx* c, void* stream) {
 ts_status ts_ctx_last_sort_info(ts_ctx* c, ts_sort_info* info, void* stream) {
   if (!c || !info) return TS_ERR_INVALID_ARG;
   std::memset(info, 0, sizeof *info);
-  if (!c->ws || c->ws_bytes < sizeof(ts::SortMeta)) return TS_OK;
+  const ts::SortMeta* m = c->last_meta;
+  if (!m) return TS_OK;
   DeviceScope ds(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const ts::SortMeta* m = reinterpret_cast<const ts::SortMeta*>(c->ws);
   uint32_t w[8];
   hipError_t e = hipMemcpyAsync(&w[0], &m->all_live, 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(&w[1], &m->n_live, 4, hipMemcpyDeviceToHost, s);
@@ -660,6 +710,38 @@ ts_status ts_sort_with_buffer(ts_ctx* c, ts_dtype dt, void* data, size_t n, void
   if (n > 1 && (!data || !buf)) return fail(c, TS_ERR_INVALID_ARG, "null data or buffer");
   DeviceScope ds(c->device);
   return sort_impl(c, dt, data, n, buf, static_cast<hipStream_t>(stream));
+}
+
+ts_status ts_sort_with_buffer_static(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, size_t buf_len,
+                                     void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(dt)) return fail(c, TS_ERR_INVALID_ARG, "unknown dtype");
+  if (buf_len < n)  // dispatch.rs:78-83
+    return fail(c, TS_ERR_BUFFER_TOO_SMALL,
+                "buffer too small: need " + std::to_string(n) + ", got " + std::to_string(buf_len));
+  if (n > kStaticMaxKeys)
+    return fail(c, TS_ERR_INVALID_ARG,
+                "ts_sort_with_buffer_static sorts at most 2^28 keys, got " + std::to_string(n));
+  if (n > 1 && (!data || !buf)) return fail(c, TS_ERR_INVALID_ARG, "null data or buffer");
+  DeviceScope ds(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int kb = ts::dtype_bytes(dt), mode = ts::dtype_mode(dt);
+  clock_reset(c);
+  if (n <= 1) return TS_OK;
+  if (n <= ts::kSmallMax) {  // one workgroup in LDS: no scratch at all
+    hipError_t e = ts::small_sort(kb, mode, data, n, nullptr, 0, 0, nullptr, s, &c->clk);
+    return e == hipSuccess ? TS_OK : hip_fail(c, e, "small_sort");
+  }
+  char* p = static_scratch(c->device);
+  if (!p) return fail(c, TS_ERR_HIP, "no static scratch address for this device");
+  if (ts::status_bytes_used(n, kb, false, false) > kStaticStatus)
+    return fail(c, TS_ERR_INVALID_ARG, "internal: static look-back status too small");
+  ts::RadixJob j = radix_job(c, kb, mode, n, p, p + kStaticMeta, kStaticStatus);
+  j.keys_a = data;
+  j.keys_b = buf;
+  j.divert = kb == 8 && n >= c->divert_min;
+  hipError_t e = ts::radix_sort(j, s, &c->clk);
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort");
 }
 
 ts_status ts_sort_host(ts_ctx* c, ts_dtype dt, void* host_data, size_t n, void* stream) {
@@ -948,6 +1030,7 @@ ts_status ts_learned_sort(ts_ctx* c, ts_dtype dt, void* data, size_t n, const ts
   j.buckets = B;
   j.offsets = reinterpret_cast<unsigned long long*>(w + off[11]);
   j.meta = reinterpret_cast<ts::SortMeta*>(w + off[0]);
+  c->last_meta = j.meta;
   j.status = w + off[1];
   j.status_cap = sb;
   j.error = c->dflags;
