@@ -43,77 +43,91 @@ static uint32_t log2_eighths(uint64_t x) {
   return 8u * e + frac;
 }
 
-// k_finish, lane by lane.  Returns whether it asked for the second sort.
-constexpr int kFT = 4096, kFH = 256, kFW = kFT + 2 * kFH, kThreads = 256, kFC = kFW / kThreads;
+// k_finish_local then k_finish, position by position, with the same tiles,
+// rounds, marks, run-start bit words and bounded word searches; tiles are
+// visited in a random order.  Returns false on a failed check; *fallback
+// tells whether the second sort was asked for.
+constexpr int kFT = 4096, kFH = 256, kFW = kFT + 2 * kFH, kFWords = kFW / 32 + 1, kFReach = kFH / 32 + 1;
+constexpr int kFRounds = 4;
 template <class U>
 static bool finish_model(std::vector<U>& a, const std::vector<U>& b, bool in_place, unsigned shift,
                          std::vector<int>& writes, std::mt19937_64& g, bool* fallback) {
   const size_t n = a.size();
-  const int kNone = 1 << 30;
   auto pre = [&](U k) { return uint64_t(k) >> shift; };
-  std::vector<size_t> order((n + kFT - 1) / kFT);
+  const size_t tiles = (n + kFT - 1) / kFT;
+  std::vector<size_t> order(tiles);
   std::iota(order.begin(), order.end(), size_t(0));
-  std::shuffle(order.begin(), order.end(), g);
   *fallback = false;
-  std::vector<U> w(kFW);
+  // k_finish_local
+  std::vector<uint32_t> marks(tiles, 0);
+  std::shuffle(order.begin(), order.end(), g);
   for (size_t tile : order) {
+    const size_t base = tile * kFT;
+    const int valid = int(std::min(n - base, size_t(kFT)));
+    const std::vector<U>& src = in_place ? a : b;
+    std::vector<U> w(src.begin() + base, src.begin() + base + valid), orig(w);
+    for (int round = 0; round < kFRounds; ++round)
+      for (int p = round & 1; p + 1 < valid; p += 2)
+        if (w[p] > w[p + 1]) std::swap(w[p], w[p + 1]);
+    bool left = false;
+    for (int q = 0; q + 1 < valid; ++q) left |= w[q] > w[q + 1];
+    const bool cross = base > 0 && pre(src[base - 1]) == pre(w[0]);
+    marks[tile] = (left ? 1u : 0u) | (cross ? 2u : 0u);
+    for (int q = 0; q < valid; ++q)
+      if (!in_place || w[q] != orig[q]) {
+        a[base + q] = w[q];
+        ++writes[base + q];
+      }
+  }
+  // k_finish on the marked tiles, in place
+  std::shuffle(order.begin(), order.end(), g);
+  std::vector<U> w(kFW);
+  std::vector<uint32_t> starts(kFWords);
+  for (size_t tile : order) {
+    if (!(marks[tile] & 3u) && !(tile + 1 < tiles && (marks[tile + 1] & 2u))) continue;
     const size_t base = tile * kFT;
     const size_t lo = base >= size_t(kFH) ? base - kFH : 0;
     const size_t hi = std::min(n, base + kFT + kFH);
     const int wn = int(hi - lo);
     const int own0 = int(base - lo), own1 = int(std::min(n, base + kFT) - lo);
-    for (int q = 0; q < wn; ++q) w[q] = in_place ? a[lo + q] : b[lo + q];
-    uint32_t bmask[kThreads];
-    int chunk_last[kThreads], chunk_first[kThreads];
-    for (int t = 0; t < kThreads; ++t) {
-      const int c0 = t * kFC;
-      bmask[t] = 0;
-      for (int r = 0; r < kFC; ++r) {
-        const int q = c0 + r;
-        if (q < wn && (q == 0 ? lo == 0 : pre(w[q]) != pre(w[q - 1]))) bmask[t] |= 1u << r;
-      }
-      chunk_last[t] = bmask[t] ? c0 + 31 - __builtin_clz(bmask[t]) : -1;
-      chunk_first[t] = bmask[t] ? c0 + __builtin_ctz(bmask[t]) : kNone;
+    for (int q = 0; q < wn; ++q) w[q] = a[lo + q];
+    std::fill(starts.begin(), starts.end(), 0u);
+    for (int q = 0; q <= kFW; ++q) {
+      bool bit;
+      if (q < wn)
+        bit = q == 0 ? lo == 0 : pre(w[q]) != pre(w[q - 1]);
+      else
+        bit = q == wn && hi == n;
+      if (bit) starts[q >> 5] |= 1u << (q & 31);
     }
-    for (int t = 0; t < kThreads; ++t) {
-      const int c0 = t * kFC;
-      int start_before = -1, start_after = kNone;
-      for (int x = 0; x < t; ++x) start_before = std::max(start_before, chunk_last[x]);
-      for (int x = t + 1; x < kThreads; ++x) start_after = std::min(start_after, chunk_first[x]);
-      const int end_default = std::min(start_after, hi == n ? wn : kNone);
-      for (int r = 0; r < kFC; ++r) {
-        const int q = c0 + r;
-        if (q < own0 || q >= wn) continue;
-        const uint32_t upto = bmask[t] & ((2u << r) - 1u), later = bmask[t] & ~((2u << r) - 1u);
-        const int rs = upto ? c0 + 31 - __builtin_clz(upto) : start_before;
-        const int re = later ? c0 + __builtin_ctz(later) : end_default;
-        const bool short_run = rs >= 0 && re != kNone && re - rs <= kFH;
+    for (int q = own0; q < wn; ++q) {
+      const bool is_start = (starts[q >> 5] >> (q & 31)) & 1u;
+      int rs = -1;
+      {
+        int wi = q >> 5;
+        uint32_t bits = starts[wi] & (0xFFFFFFFFu >> (31 - (q & 31)));
+        for (int k = 0; !bits && wi > 0 && k < kFReach; ++k) bits = starts[--wi];
+        if (bits) rs = wi * 32 + 31 - __builtin_clz(bits);
+      }
+      int re = -1;
+      if (rs >= 0 && q - rs < kFH) {
+        const int q1 = q + 1;
+        int wi = q1 >> 5;
+        uint32_t bits = starts[wi] & (0xFFFFFFFFu << (q1 & 31));
+        for (int k = 0; !bits && wi < kFWords - 1 && k < kFReach; ++k) bits = starts[++wi];
+        if (bits) re = wi * 32 + __builtin_ctz(bits);
+      }
+      const bool short_run = rs >= 0 && re >= 0 && re - rs <= kFH;
+      if (short_run) {
+        if (rs < own0 || rs >= own1 || re - rs == 1) continue;
         const U k = w[q];
-        if (short_run) {
-          if (rs < own0 || rs >= own1) continue;
-          if (re - rs == 1) {
-            if (!in_place) {
-              a[lo + q] = k;
-              ++writes[lo + q];
-            }
-            continue;
-          }
-          int rank = 0;
-          for (int j = rs; j < re; ++j) rank += (w[j] < k) || (w[j] == k && j < q);
-          const int o = rs + rank;
-          CHECK(o >= rs && o < re, "finish rank %d outside [%d, %d)", o, rs, re);
-          if (!in_place || o != q) {
-            a[lo + o] = k;
-            ++writes[lo + o];
-          }
-        } else {
-          if (q >= own1) continue;
-          if (!in_place) {
-            a[lo + q] = k;
-            ++writes[lo + q];
-          }
-          if (rs != q && w[q - 1] > k) *fallback = true;
-        }
+        int rank = 0;
+        for (int j = rs; j < re; ++j) rank += (w[j] < k) || (w[j] == k && j < q);
+        const int o = rs + rank;
+        CHECK(o >= rs && o < re, "finish rank %d outside [%d, %d)", o, rs, re);
+        if (o != q) a[lo + o] = k;
+      } else if (q < own1 && !is_start && w[q - 1] > w[q]) {
+        *fallback = true;
       }
     }
   }
@@ -260,7 +274,7 @@ bool model(const std::vector<U>& keys, size_t tile, const char* name, bool diver
   if (!finish_model(a, b, in_place, shift, writes, g, &fallback)) return false;
   fallback_cases += fallback;
   if (!in_place)
-    for (size_t i = 0; i < n; ++i) CHECK(writes[i] == 1, "%s: finish writes %zu %d times", name, i, writes[i]);
+    for (size_t i = 0; i < n; ++i) CHECK(writes[i] == 1, "%s: local finish writes %zu %d times", name, i, writes[i]);
   // Expected: the second sort is needed exactly when a run longer than the
   // halo is out of order.
   bool long_unsorted = false;

@@ -83,9 +83,12 @@ def check(x: np.ndarray, dev, ctx, what: str, buffer=False) -> dict:
     ctx.check_device()
     if x.size > SMALL_N and x.size >= ctx.divert_min:
         live, skip, second = expected_divert(x)
-        assert (info["live_digits"], info["skipped_digits"], info["fallback"]) == (live, skip, second), (
-            what, info, (live, skip, second))
-        assert info["executed_passes"] == (live if second else live - skip), info
+        assert (info["live_digits"], info["skipped_digits"]) == (live, skip), (what, info, (live, skip, second))
+        # The second sort runs only when a run too long for the finish is out
+        # of order (the finish's first rounds may still put a nearly sorted
+        # one in order).
+        assert info["fallback"] <= second, (what, info, (live, skip, second))
+        assert info["executed_passes"] == (live if info["fallback"] else live - skip), info
     return info
 
 

@@ -823,119 +823,230 @@ __global__ void k_iota_if_dead(const SortMeta* m, uint32_t* out, size_t n) {
 // ---------------------------------------------------------------- finish
 // After a divert the keys are in order of their bits from finish_shift up
 // (the prefix), and each run of keys sharing a prefix only needs sorting
-// within itself (finish_runs, divert.rs:60-90).  A workgroup takes a tile of
-// kFinishTile positions and loads them with kFinishHalo keys on either side
-// into LDS.  Every run of at most kFinishHalo keys is sorted by the tile that
-// holds its first key: each key's rank is the number of keys of its run that
-// are smaller, or equal and earlier (the order among equal keys is not
-// observable).  A longer run is left as it is when it is already in order (a
-// run of one repeated key, say); otherwise the finish sets m->fallback and
-// the gated second sort sorts every digit.  Every tile classifies every run
-// the same way (short runs lie inside the window of each tile they touch).
+// within itself (finish_runs, divert.rs:60-90).  Two kernels do it.
 //
-// The data is in the first buffer after an even number of executed passes:
-// the finish then works in place and writes only keys that move; a tile
-// never writes a run it does not own, and what it reads of other runs is
-// only their prefixes, which moves inside a run do not change.  After an odd
-// number it moves every key from the second buffer into the first.
+// k_finish_local: each workgroup takes tiles of kFinishTile positions (no
+// overlap) and runs kLocalRounds rounds of odd-even transposition on the
+// tile in LDS, comparing whole keys.  Neighbouring keys of different runs
+// are already in order, so only keys of one run ever swap, and every run of
+// at most kLocalRounds keys inside the tile comes out sorted.  The tile then
+// records (one word per tile, in the look-back status area, free after the
+// scatter passes) whether an out-of-order pair is left inside it (bit 0),
+// and whether a run crosses its first position, so that the tile before it
+// and it must be looked at again (bit 1).  It reads the position before its
+// tile only for that run's prefix.
+//
+// k_finish: the tiles marked above (and the tile before a bit-1 tile) are
+// finished again with windows of kFinishHalo keys on either side.  Every run
+// of at most kFinishHalo keys is sorted by the tile that holds its first
+// key: each key's rank is the number of keys of its run that are smaller, or
+// equal and earlier (the order among equal keys is not observable).  A
+// longer run is left as it is when it is already in order (a run of one
+// repeated key, say); otherwise the finish sets m->fallback and the gated
+// second sort sorts every digit.  Every tile classifies every run the same
+// way (short runs lie inside the window of each tile they touch).  Run
+// starts are kept as one bit per window position (wave ballots).
+//
+// The workgroups of one XCD (dispatch deals workgroups round-robin over the
+// 8 XCDs) take consecutive tiles of one eighth of the array, so that
+// neighbouring tiles are read through one L2.  The data is in the first
+// buffer after an even number of executed passes: both kernels then work in
+// place and write only keys that move; a k_finish tile never writes a run it
+// does not own, and what it reads of other runs is only their prefixes,
+// which moves inside a run do not change.  After an odd number the local
+// finish moves every key from the second buffer into the first.
 constexpr int kFinishThreads = 256;
 constexpr int kFinishTile = kFinishThreads * 16;
+constexpr int kFinishPer = kFinishTile / kFinishThreads;
+constexpr int kLocalRounds = 4;
 constexpr int kFinishHalo = 256;
 constexpr int kFinishWin = kFinishTile + 2 * kFinishHalo;
-constexpr int kFinishChunk = kFinishWin / kFinishThreads;  // window positions per lane
-static_assert(kFinishWin % kFinishThreads == 0, "whole chunks");
-static_assert(kFinishHalo < kFinishTile, "a short run touches at most two tiles");
+constexpr int kFinishSteps = kFinishWin / kFinishThreads;  // window positions per lane
+constexpr int kFinishWords = kFinishWin / 32 + 1;  // run-start bits, and the bit just past the window
+constexpr int kFinishReach = kFinishHalo / 32 + 1;  // words a short run's bound can lie away
+static_assert(kFinishWin % kFinishThreads == 0 && kFinishThreads % 64 == 0, "whole steps");
+static_assert(kFinishHalo < kFinishTile && kFinishHalo % 32 == 0, "a short run touches at most two tiles");
 
-// Exclusive scan of one int per lane over the workgroup: the max over the
-// earlier lanes (FWD) or the min over the later lanes, `id` when there are
-// none.  s holds one slot per wave.
-template <bool FWD> __device__ __forceinline__ int block_excl_extreme(int v, int id, int* s) {
-  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  constexpr unsigned NW = kFinishThreads / 64;
-  int incl = v;
-#pragma unroll
-  for (unsigned d = 1; d < 64; d <<= 1) {
-    const int o = FWD ? __shfl_up(incl, d, 64) : __shfl_down(incl, d, 64);
-    if (FWD ? lane >= d : lane + d < 64u) incl = FWD ? max(incl, o) : min(incl, o);
-  }
-  int ex = FWD ? __shfl_up(incl, 1u, 64) : __shfl_down(incl, 1u, 64);
-  if (lane == (FWD ? 0u : 63u)) ex = id;
-  if (lane == (FWD ? 63u : 0u)) s[w] = incl;
-  __syncthreads();
-#pragma unroll
-  for (unsigned x = 0; x < NW; ++x)
-    if (FWD ? x < w : x > w) ex = FWD ? max(ex, s[x]) : min(ex, s[x]);
-  __syncthreads();
-  return ex;
-}
+// Bytes of per-tile marks the local finish writes into the status area.
+inline size_t finish_mark_bytes(size_t n) { return ((n + kFinishTile - 1) / kFinishTile) * sizeof(uint32_t); }
 
 template <class U>
-__global__ __launch_bounds__(kFinishThreads) void k_finish(SortMeta* m, U* __restrict__ keys_a,
-                                                           const U* __restrict__ keys_b, size_t n, int mode) {
+__global__ __launch_bounds__(kFinishThreads) void k_finish_local(SortMeta* m, U* __restrict__ keys_a,
+                                                                 const U* __restrict__ keys_b, size_t n, int mode,
+                                                                 uint32_t* __restrict__ marks) {
   if (m->skip == 0) return;
   const uint32_t shift = m->finish_shift;
   const bool in_place = (m->n_live & 1u) == 0;
   const U* src = in_place ? keys_a : keys_b;
   U* dst = keys_a;
-  __shared__ U w[kFinishWin];
-  __shared__ int s_scan[kFinishThreads / 64];
+  __shared__ U w[kFinishTile];
   const unsigned tid = threadIdx.x;
-  constexpr int kNone = 1 << 30;
+  const size_t tiles = (n + kFinishTile - 1) / kFinishTile;
+  const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+  const size_t stretch = (tiles + 7) / 8;
+  const size_t t_end = min(tiles, size_t(xcd + 1) * stretch);
+  for (size_t tile = size_t(xcd) * stretch + blockIdx.x / 8; tile < t_end; tile += per_xcd) {
+    const size_t base = tile * kFinishTile;
+    const int valid = int(min(n - base, size_t(kFinishTile)));
+    const U* __restrict__ b = src + base;  // uniform: one 32-bit offset per load
+    U reg[kFinishPer];
+#pragma unroll
+    for (int r = 0; r < kFinishPer; ++r) reg[r] = b[min(r * kFinishThreads + int(tid), valid - 1)];
+    to_keys(reg, mode);
+    __syncthreads();  // the previous tile's keys are no longer read
+#pragma unroll
+    for (int r = 0; r < kFinishPer; ++r) {
+      const int q = r * kFinishThreads + int(tid);
+      if (q < valid) w[q] = reg[r];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int round = 0; round < kLocalRounds; ++round) {
+#pragma unroll
+      for (int r = 0; r < kFinishPer / 2; ++r) {
+        const int p = 2 * (r * kFinishThreads + int(tid)) + (round & 1);
+        if (p + 1 < valid) {
+          const U x = w[p], y = w[p + 1];
+          if (x > y) {
+            w[p] = y;
+            w[p + 1] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    bool left = false;
+#pragma unroll
+    for (int r = 0; r < kFinishPer; ++r) {
+      const int q = r * kFinishThreads + int(tid);
+      if (q + 1 < valid && w[q] > w[q + 1]) left = true;
+    }
+    left = __syncthreads_or(left);
+    if (tid == 0) {
+      // A run crossing into this tile: compare prefixes only (the tile
+      // before may be moving keys inside that run right now).
+      const bool cross = base > 0 && (uint64_t(to_key(src[base - 1], mode)) >> shift) == (uint64_t(w[0]) >> shift);
+      marks[tile] = (left ? 1u : 0u) | (cross ? 2u : 0u);
+    }
+#pragma unroll
+    for (int r = 0; r < kFinishPer; ++r) {
+      const int q = r * kFinishThreads + int(tid);
+      if (q < valid) {
+        const U v = w[q];
+        if (!in_place || v != reg[r]) dst[base + size_t(q)] = from_key(v, mode);
+      }
+    }
+  }
+}
+
+template <class U>
+__global__ __launch_bounds__(kFinishThreads) void k_finish(SortMeta* m, U* __restrict__ keys_a, size_t n, int mode,
+                                                           const uint32_t* __restrict__ marks) {
+  if (m->skip == 0) return;
+  const uint32_t shift = m->finish_shift;
+  // After the local finish the keys are in the first buffer.
+  U* src = keys_a;
+  U* dst = keys_a;
+  __shared__ U w[kFinishWin];
+  __shared__ uint32_t starts[kFinishWords];
+  const unsigned tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   auto pre = [&](U k) { return uint64_t(k) >> shift; };
   bool unsorted = false;
-  for (size_t tile = blockIdx.x; tile * kFinishTile < n; tile += gridDim.x) {
+  const size_t tiles = (n + kFinishTile - 1) / kFinishTile;
+  const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+  const size_t stretch = (tiles + 7) / 8;
+  const size_t t_end = min(tiles, size_t(xcd + 1) * stretch);
+  for (size_t tile = size_t(xcd) * stretch + blockIdx.x / 8; tile < t_end; tile += per_xcd) {
+    // Only marked tiles, and a tile whose successor's first run crosses
+    // back into it.
+    if (!(marks[tile] & 3u) && !(tile + 1 < tiles && (marks[tile + 1] & 2u))) continue;  // uniform
     const size_t base = tile * kFinishTile;
     const size_t lo = base >= size_t(kFinishHalo) ? base - kFinishHalo : 0;
     const size_t hi = min(n, base + kFinishTile + kFinishHalo);
     const int wn = int(hi - lo);
     const int own0 = int(base - lo), own1 = int(min(n, base + kFinishTile) - lo);  // this tile: [own0, own1)
-    __syncthreads();  // the previous tile's window is no longer read
-    for (int q = int(tid); q < wn; q += kFinishThreads) w[q] = to_key(src[lo + size_t(q)], mode);
-    __syncthreads();
-    // Run starts in this lane's chunk of the window: position q starts a run
-    // when its prefix differs from q-1's (window position 0 only at the
-    // array's start; before the window nothing is known).
-    const int c0 = int(tid) * kFinishChunk;
-    uint32_t bmask = 0;
+    U reg[kFinishSteps];
+    {
+      const U* __restrict__ b = src + lo;
 #pragma unroll
-    for (int r = 0; r < kFinishChunk; ++r) {
-      const int q = c0 + r;
-      if (q < wn && (q == 0 ? lo == 0 : pre(w[q]) != pre(w[q - 1]))) bmask |= 1u << r;
+      for (int r = 0; r < kFinishSteps; ++r) reg[r] = b[min(r * kFinishThreads + int(tid), wn - 1)];
     }
-    const int chunk_last = bmask ? c0 + 31 - __clz(bmask) : -1;
-    const int chunk_first = bmask ? c0 + __ffs(bmask) - 1 : kNone;
-    const int start_before = block_excl_extreme<true>(chunk_last, -1, s_scan);
-    const int start_after = block_excl_extreme<false>(chunk_first, kNone, s_scan);
-    // A run with no start after it inside the window ends at the window's
-    // end if that is the array's end, else somewhere unknown.
-    const int end_default = min(start_after, hi == n ? wn : kNone);
+    to_keys(reg, mode);
+    __syncthreads();  // the previous tile's window is no longer read
+#pragma unroll
+    for (int r = 0; r < kFinishSteps; ++r) {
+      const int q = r * kFinishThreads + int(tid);
+      if (q < wn) w[q] = reg[r];
+    }
+    if (tid == 0) starts[kFinishWords - 1] = (wn == kFinishWin && hi == n) ? 1u : 0u;
+    __syncthreads();
+    // Bit q: a run starts at window position q (its prefix differs from
+    // q-1's; position 0 only at the array's start), or q = wn is the array's
+    // end.  Before the window nothing is known.
+#pragma unroll 3
+    for (int r = 0; r < kFinishSteps; ++r) {
+      const int q = r * kFinishThreads + int(tid);
+      bool bt;
+      if (q < wn)
+        bt = q == 0 ? lo == 0 : pre(w[q]) != pre(w[q - 1]);
+      else
+        bt = q == wn && hi == n;
+      const uint64_t bal = __ballot(bt);
+      if (lane == 0) {
+        const int wd = (r * kFinishThreads + int(wave) * 64) >> 5;
+        starts[wd] = uint32_t(bal);
+        starts[wd + 1] = uint32_t(bal >> 32);
+      }
+    }
+    __syncthreads();
+    // Each wave takes 64 consecutive positions per step; their run-start
+    // bits are one 64-bit word every lane reads (a broadcast).
 #pragma unroll 1
-    for (int r = 0; r < kFinishChunk; ++r) {
-      const int q = c0 + r;
+    for (int r = 0; r < kFinishSteps; ++r) {
+      const int q0 = r * kFinishThreads + int(wave) * 64, q = q0 + int(lane);
+      if (q0 + 64 <= own0 || q0 >= wn) continue;  // the whole wave is outside: uniform
+      const int wd = q0 >> 5;
+      const uint64_t B = uint64_t(starts[wd]) | uint64_t(starts[wd + 1]) << 32;
+      const uint64_t upto = B & ((2ull << lane) - 1ull);  // bits 0 .. lane (all for lane 63)
+      const uint64_t later = B & ~((2ull << lane) - 1ull);
       if (q < own0 || q >= wn) continue;
-      const uint32_t upto = bmask & ((2u << r) - 1u), later = bmask & ~((2u << r) - 1u);
-      const int rs = upto ? c0 + 31 - __clz(upto) : start_before;  // run start (-1: before the window)
-      const int re = later ? c0 + __ffs(later) - 1 : end_default;  // run end (kNone: past the window)
-      const bool short_run = rs >= 0 && re != kNone && re - rs <= kFinishHalo;
-      const U k = w[q];
-      if (short_run) {
-        if (rs < own0 || rs >= own1) continue;  // another tile's run
-        if (re - rs == 1) {
-          if (!in_place) dst[lo + size_t(q)] = from_key(k, mode);
-          continue;
+      const bool is_start = (B >> lane) & 1u;
+      // The run's start: the last start bit at or before q.
+      int rs = -1;
+      if (upto) {
+        rs = q0 + 63 - int(__clzll((long long)upto));
+      } else {
+        int wi = wd;
+        uint32_t bitsw = 0;
+        for (int k = 0; !bitsw && wi > 0 && k < kFinishReach; ++k) bitsw = starts[--wi];
+        if (bitsw) rs = wi * 32 + 31 - __clz(bitsw);
+      }
+      // Its end: the first start bit after q.
+      int re = -1;
+      if (rs >= 0 && q - rs < kFinishHalo) {
+        if (later) {
+          re = q0 + __ffsll((unsigned long long)later) - 1;
+        } else {
+          int wi = wd + 1;
+          uint32_t bitsw = 0;
+          for (int k = 0; !bitsw && wi < kFinishWords - 1 && k < kFinishReach; ++k) bitsw = starts[++wi];
+          if (bitsw) re = wi * 32 + __ffs(bitsw) - 1;
         }
+      }
+      const bool short_run = rs >= 0 && re >= 0 && re - rs <= kFinishHalo;
+      if (short_run) {
+        if (rs < own0 || rs >= own1 || re - rs == 1) continue;  // another tile's run, or one key
+        const U k = w[q];
         int rank = 0;
         for (int j = rs; j < re; ++j) {
           const U x = w[j];
           rank += (x < k) || (x == k && j < q);
         }
         const int o = rs + rank;
-        if (!in_place || o != q) dst[lo + size_t(o)] = from_key(k, mode);
-      } else {
-        if (q >= own1) continue;  // the next tile's position
-        if (!in_place) dst[lo + size_t(q)] = from_key(k, mode);
-        // q is not the run's start, so q - 1 lies in the same run.
-        if (rs != q && w[q - 1] > k) unsorted = true;
+        if (o != q) dst[lo + size_t(o)] = from_key(k, mode);
+      } else if (q < own1 && !is_start && w[q - 1] > w[q]) {
+        // A long run out of order (q is not its start, so q - 1 lies in it).
+        unsorted = true;
       }
     }
   }
@@ -949,10 +1060,18 @@ size_t first_segment(size_t n, int key_bytes, bool learned, bool vals) {
   return ((per + tile - 1) / tile) * tile;
 }
 
+// Keys-only sorts of 8-byte keys that may divert launch their passes from
+// this one on as persistent grids: after a divert they do not run, and a
+// persistent grid ends in microseconds where a grid of one workgroup per
+// tile takes about 11 us at 2^28 keys.
+#ifndef TS_SWEEP_PERSIST_FROM
+#define TS_SWEEP_PERSIST_FROM 4
+#endif
+
 // gate: the second sort after a divert (persistent launches for keys-only
 // 8-byte keys, run only when the finish found an out-of-order long run).
 template <class U, class S, bool LEARNED>
-hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool gate) {
+hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool gate, bool divert = false) {
   const bool vals = j.final_vals != nullptr;
   const uint32_t tiles = sweep_tiles(j.n, sizeof(U), LEARNED, vals);
   SweepArgs<U> a;
@@ -981,7 +1100,7 @@ hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool g
     } else if (vals) {
       hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U), false, true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
-    } else if (gate) {
+    } else if (gate || (divert && p >= TS_SWEEP_PERSIST_FROM)) {
       if constexpr (sizeof(U) == 8) {
         const unsigned grid = unsigned(std::min<size_t>(tiles, size_t(j.num_cus)));
         hipLaunchKernelGGL((k_sweep<U, false, S, false, false, true>), dim3(grid), dim3(SweepShape<sizeof(U)>::threads),
@@ -1060,16 +1179,23 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   };
   hist(0);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, s, j.meta, j.n, P, seg0, tile, check_order, divert ? 1 : 0, 0);
-  hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk, false)
-                                  : run_sweeps<U, uint32_t, LEARNED>(j, s, clk, false);
+  hipError_t e = wide_status(j.n) ? run_sweeps<U, uint64_t, LEARNED>(j, s, clk, false, divert)
+                                  : run_sweeps<U, uint32_t, LEARNED>(j, s, clk, false, divert);
   if (e != hipSuccess) return e;
   if (divert) {
     if constexpr (!LEARNED && KB == 8) {
       // Finish the runs, then the second sort that runs only if a long run
-      // was out of order.
-      if (clk) clk->begin(s, "k_finish", double(j.n) * KB);
-      hipLaunchKernelGGL((k_finish<U>), dim3(unsigned(j.num_cus) * 4), dim3(kFinishThreads), 0, s, j.meta,
-                         static_cast<U*>(j.keys_a), static_cast<const U*>(j.keys_b), j.n, j.mode);
+      // was out of order.  The per-tile marks live in the status area.
+      if (status_bytes_used(j.n, KB, false, false) < finish_mark_bytes(j.n)) return hipErrorInvalidValue;
+      uint32_t* marks = static_cast<uint32_t*>(j.status);
+      const unsigned fgrid = std::max(8u, (unsigned(j.num_cus) * 4u) & ~7u);  // a multiple of 8 (XCDs)
+      if (clk) clk->begin(s, "k_finish_local", double(j.n) * KB);
+      hipLaunchKernelGGL((k_finish_local<U>), dim3(fgrid), dim3(kFinishThreads), 0, s, j.meta,
+                         static_cast<U*>(j.keys_a), static_cast<const U*>(j.keys_b), j.n, j.mode, marks);
+      if (clk) clk->end(s);
+      if (clk) clk->begin(s, "k_finish", 0.0);
+      hipLaunchKernelGGL((k_finish<U>), dim3(fgrid), dim3(kFinishThreads), 0, s, j.meta, static_cast<U*>(j.keys_a),
+                         j.n, j.mode, static_cast<const uint32_t*>(marks));
       if (clk) clk->end(s);
       hipLaunchKernelGGL(k_init, dim3(16), dim3(256), 0, s, j.meta, P, 1);
       hist(1);
