@@ -8,7 +8,10 @@ region; the K steps run back to back between two synchronizations.
 Printed: one JSON line with the driver's fields plus ``roofline`` (the
 one-sweep scatter kernel, timed with HIP events on its own stream) and
 ``cpu_baseline`` (the C++ restatement of turbosort::sort_parallel on this
-host's cores, on a bounded sample).
+host's CPU share, on a bounded sample).  ``also`` holds every other workload
+the same way: device-resident rate, rate with the copies from and to pinned
+host memory, roofline fractions, and a CPU baseline on a sample of the same
+input.
 """
 from __future__ import annotations
 
@@ -38,14 +41,82 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="u32", choices=sorted(WORKLOADS))
     p.add_argument("--log2n", type=int, default=28)
-    p.add_argument("--also", default="u64,u64z,host,segmented,by_key,learned",
-                   help="extra workloads reported beside the headline (u32, u64, u64z, host, segmented, by_key, learned)")
-    p.add_argument("--cpu-seconds", type=float, default=8.0, help="wall budget of the CPU baseline")
+    p.add_argument("--also", default="u64,u64z,segmented,by_key,learned",
+                   help="extra workloads reported beside the headline (u32, u64, u64z, segmented, by_key, learned)")
+    p.add_argument("--cpu-seconds", type=float, default=2.5,
+                   help="wall budget of each CPU baseline (one per workload)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-host", action="store_true", help="skip the rates that include host copies")
+    p.add_argument("--learned-n", type=int, default=1_000_000_000)
     return p.parse_args()
 
 
-def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx):
+def median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2]
+
+
+# ------------------------------------------------------------ CPU baseline
+def cpu_share():
+    """Threads the CPU baselines use: the CPU share this process is given
+    (OMP_NUM_THREADS where the host sets it, else the CPUs it may run on),
+    with the host's CPU count and model for the record."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else affinity
+    threads = max(1, min(threads, affinity))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(threads=threads, nproc=os.cpu_count(), affinity=affinity, model=model)
+
+
+def cpu_timed(prepare, run, budget_s: float, min_runs: int = 3, max_runs: int = 40):
+    """Median wall time of run() after prepare() (untimed), repeated within a budget."""
+    prepare()
+    run()  # warm-up
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while (time.perf_counter() < t_end or len(times) < min_runs) and len(times) < max_runs:
+        prepare()
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return median(times), len(times)
+
+
+def cpu_sort_baseline(src, threads: int, budget_s: float, what: str):
+    """The C++ restatement of turbosort::sort_parallel on a copy of src."""
+    import numpy as np
+
+    from oracle import native as nv
+
+    work = np.empty_like(src)
+    t, runs = cpu_timed(lambda: np.copyto(work, src), lambda: nv.sort_parallel_inplace(work, threads), budget_s)
+    return dict(value=src.size / t, unit="keys/s", cores=threads, kind="port",
+                sample=f"{what}: {src.size} keys, median of {runs} sorts by the C++ restatement of "
+                       f"turbosort::sort_parallel (parallel.rs:46-291) with {threads} threads")
+
+
+# ------------------------------------------------------------ plain sorts
+def cpu_sample(name: str, n: int):
+    from oracle import closed_form as cf
+
+    _, kind, seed, _, _ = WORKLOADS[name]
+    return (cf.u32_keys(seed, n) if kind == 0 else cf.u64_keys(seed, n) if kind == 1
+            else cf.u64_zero_extended_u32(seed, n))
+
+
+def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx, host: bool = True, cpu=None):
     import torch
 
     import turbosort_amd as ts
@@ -79,12 +150,14 @@ def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     per = sorted(a.elapsed_time(b) for a, b in ev)
+    info = ctx.last_sort_info()
 
     # The result must be the sorted permutation of the input.
     after = ts.checksum(bufs[-1], ctx=ctx)
     ok = after[0] == before[0] and after[1] == before[1] and after[2] == 0
 
-    # Per-kernel durations from HIP events on the sort's stream.
+    # Per-kernel durations from HIP events on the sort's stream; the executed
+    # scatter passes are the first ones launched.
     ctx.set_profiling(True)
     kt = []
     for i in range(3):
@@ -94,25 +167,98 @@ def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx):
         torch.cuda.synchronize()
         kt.append(ctx.kernel_times())
     ctx.set_profiling(False)
-    sweep = [t for run in kt for (nm, t, _b) in run if nm == "k_sweep"]
-    sweep.sort(reverse=True)
-    sweep_live = sweep[: live * len(kt)]
+    n_exec = info["executed_passes"]
+    sweep_live = [t for run in kt for (_nm, t, _b) in [x for x in run if x[0] == "k_sweep"][:n_exec]]
     hist = [t for run in kt for (nm, t, _b) in run if nm == "k_hist"]
+    finish = [sum(t for (nm, t, _b) in run if nm.startswith("k_finish")) for run in kt]
     sweep_ms = sum(sweep_live) / len(sweep_live)
     hist_ms = sum(hist) / len(hist)
     sweep_bytes = 2.0 * w * n  # one read and one write of every key
     algo_bytes = w * (1 + 2 * live) * n  # canonical LSD: one histogram read + P read/write passes
-
+    # What this sort reads and writes as executed: the histogram read, the
+    # executed passes, and after a divert the finish's read of every key.
+    exec_bytes = w * (1 + 2 * n_exec + (1 if info["skipped_digits"] and not info["fallback"] else 0)) * n
     del bufs
     torch.cuda.empty_cache()
     ms_step = wall * 1e3 / steps
-    return dict(
+    out = dict(
         name=name, desc=desc, n=n, w=w, live=live, ok=ok, ms_per_step=ms_step,
         event_ms_median=per[len(per) // 2], event_ms_best=per[0],
         keys_per_s=n / (wall / steps), sweep_ms=sweep_ms, hist_ms=hist_ms,
+        finish_ms=median(finish) if info["skipped_digits"] else 0.0,
         sweep_gbs=sweep_bytes / (sweep_ms * 1e-3) / 1e9, hist_gbs=w * n / (hist_ms * 1e-3) / 1e9,
         algo_bytes_per_key=w * (1 + 2 * live), total_gbs=algo_bytes / (wall / steps) / 1e9,
+        roofline_frac=algo_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+        executed=dict(passes=n_exec, skipped_digits=info["skipped_digits"], second_sort=bool(info["fallback"]),
+                      bytes_per_key=exec_bytes / n,
+                      roofline_frac=exec_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS),
     )
+    if host:
+        out["with_copies"] = run_host(name, log2n, max(3, steps // 3), ctx)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu_sort_baseline(cpu_sample(name, 1 << 25), cpu["threads"], cpu["budget"],
+                                                f"first 2^25 keys of the same stream ({desc.split(', ', 1)[1]})")
+    return out
+
+
+def run_host(name: str, log2n: int, steps: int, ctx):
+    """sort_host on the workload's keys in pinned host memory: copy to the
+    device, sort, copy back; the end-to-end rate for a caller whose slice
+    lives in host memory (wall time, median)."""
+    import torch
+
+    import turbosort_amd as ts
+
+    dt, kind, seed, _, _ = WORKLOADS[name]
+    tdt = getattr(torch, dt)
+    n = 1 << log2n
+    dev = torch.device("cuda", 0)
+    d = torch.empty(n, dtype=tdt, device=dev)
+    if kind == 2:
+        ts.fill_splitmix64_zero_extended(d, seed, ctx=ctx)
+    else:
+        ts.fill_splitmix64(d, seed, ctx=ctx)
+    src = torch.empty(n, dtype=tdt, pin_memory=True)
+    src.copy_(d)
+    before = ts.checksum(d, ctx=ctx)
+    h = torch.empty_like(src).pin_memory()
+    h.copy_(src)
+    ts.sort_host(h, ctx=ctx)  # warm: the context's device buffer grows once
+    times = []
+    for _ in range(steps):
+        h.copy_(src)
+        t0 = time.perf_counter()
+        ts.sort_host(h, ctx=ctx)
+        times.append(time.perf_counter() - t0)
+    d.copy_(h)
+    after = ts.checksum(d, ctx=ctx)
+    ok = after[0] == before[0] and after[1] == before[1] and after[2] == 0
+    t = median(times)
+    w = d.element_size()
+    del d, src, h
+    torch.cuda.empty_cache()
+    return dict(desc="sort_host: the keys in pinned host memory, copied to the device, sorted, copied back "
+                     "(wall time, median)", ms_per_step=t * 1e3, keys_per_s=n / t,
+                link_GBps_both_ways=2 * w * n / t / 1e9, ok=ok)
+
+
+def host_roundtrip(pairs, fn, steps: int):
+    """Median wall time of: copy each pinned host tensor to its device tensor,
+    fn(), copy every device tensor back, wait.  pairs: (host, device)."""
+    import torch
+
+    times = []
+    for _ in range(steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for h, d in pairs:
+            d.copy_(h, non_blocking=True)
+        fn()
+        for h, d in pairs:
+            h.copy_(d, non_blocking=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    return median(times[1:])
 
 
 # ------------------------------------------------------------ generators
@@ -196,7 +342,7 @@ def timed(fn, steps: int):
     return (time.perf_counter() - t0) / steps
 
 
-def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7):
+def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7, host: bool = True, cpu=None):
     """Batched segmented key-value sort: 10^5 log-uniform segments in [2, 16384],
     u32 keys uniform (SplitMix64 state seed + 2^32), u32 values = global index."""
     import torch
@@ -207,7 +353,8 @@ def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7):
     dev = torch.device("cuda", 0)
     # Kernel shape per size class, picked on this GPU before the timed runs.
     tuning = autotune.tune_segmented(torch.uint32, True, ctx=ctx)
-    off = torch.from_numpy(segment_offsets(seed, n_segments)).to(dev)
+    off_np = segment_offsets(seed, n_segments)
+    off = torch.from_numpy(off_np).to(dev)
     n = int(off[-1])
     src = torch.empty(n, dtype=torch.uint32, device=dev)
     ts.fill_splitmix64(src, seed + (1 << 32), ctx=ctx)
@@ -250,17 +397,49 @@ def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7):
         torch.cuda.synchronize()
         times.append(e0.elapsed_time(e1))
     times.sort()
-    del gk, gv, graph
+    del graph
+    out = dict(desc="batched segmented key-value sort, 10^5 log-uniform segments in [2, 16384] (SplitMix64 "
+                    "state 7), u32 keys uniform, u32 values = global index", n=n, segments=n_segments,
+               ms_per_step=dt * 1e3, pairs_per_s=n / dt, algo_bytes_per_pair=16,
+               algorithmic_GBps=16 * n / dt / 1e9, roofline_frac=16 * n / dt / 1e9 / HBM_PEAK_GBS,
+               graph_replay_ms_median=times[len(times) // 2],
+               autotuned={str(c): tuning["names"][c][v] for c, v in tuning["choice"].items()}, ok=ok)
+    if host:
+        hk = torch.empty(n, dtype=torch.uint32, pin_memory=True)
+        hv = torch.empty(n, dtype=torch.int32, pin_memory=True)
+        hk.copy_(src)
+        hv.copy_(torch.arange(n, dtype=torch.int32, device=dev))
+        t = host_roundtrip([(hk, gk), (hv, gv)],
+                           lambda: ts.segmented_sort(gk, off, gv, max_segment_len=16384, ctx=ctx), max(3, steps // 2))
+        out["with_copies"] = dict(desc="keys and values in pinned host memory: copied in, sorted, copied out "
+                                       "(offsets resident; wall time, median)", ms_per_step=t * 1e3,
+                                  pairs_per_s=n / t)
+        del hk, hv
+    del gk, gv
     torch.cuda.empty_cache()
-    return dict(desc="batched segmented key-value sort, 10^5 log-uniform segments in [2, 16384] (SplitMix64 "
-                     "state 7), u32 keys uniform, u32 values = global index", n=n, segments=n_segments,
-                ms_per_step=dt * 1e3, pairs_per_s=n / dt, algo_bytes_per_pair=16,
-                algorithmic_GBps=16 * n / dt / 1e9, roofline_frac=16 * n / dt / 1e9 / HBM_PEAK_GBS,
-                graph_replay_ms_median=times[len(times) // 2],
-                autotuned={str(c): tuning["names"][c][v] for c, v in tuning["choice"].items()}, ok=ok)
+    if cpu is not None:
+        import numpy as np
+
+        from oracle import native as nv
+
+        m = 10_000  # the first 10^4 segments of the same workload
+        sub = off_np[: m + 1].astype(np.uint64)
+        keys0 = src[: int(sub[-1])].view(torch.int32).cpu().numpy().view(np.uint32)
+        kw, vw = np.empty_like(keys0), np.empty(keys0.size, dtype=np.uint32)
+
+        def prep():
+            np.copyto(kw, keys0)
+            vw[:] = np.arange(keys0.size, dtype=np.uint32)
+
+        t, runs = cpu_timed(prep, lambda: nv.segmented_sort_pairs_inplace(kw, sub, vw, cpu["threads"]), cpu["budget"])
+        out["cpu_baseline"] = dict(value=keys0.size / t, unit="pairs/s", cores=cpu["threads"], kind="port",
+                                   sample=f"the first {m} segments ({keys0.size} pairs), median of {runs} runs of the "
+                                          "restated per-segment sort_by_key_with_buffer + move (by_key.rs:175-235, "
+                                          f"315-334) with {cpu['threads']} threads taking segments in turn")
+    return out
 
 
-def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5):
+def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5, host: bool = True, cpu=None):
     """Stable sort_by_key of 10^8 16-byte records keyed on an f32 score at
     offset 4, then apply_order on a second u64 slice (SplitMix64 state 6)."""
     import numpy as np
@@ -271,6 +450,7 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5):
     dev = torch.device("cuda", 0)
     rec = score_records(n, seed)
     src = torch.from_numpy(rec.view(np.uint8).reshape(n, 16)).to(dev)
+    scores_sample = rec["score"][: 10_000_000].copy().view(np.float32) if cpu is not None else None
     del rec
     other_src = torch.empty(n, dtype=torch.uint64, device=dev)
     ts.fill_splitmix64(other_src, 6, ctx=ctx)
@@ -295,47 +475,44 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5):
           and not bool(((rk[1:] == rk[:-1]) & (order[1:] <= order[:-1])).any())
           and bool(torch.equal(rs[-1], take(src, order)))
           and bool(torch.equal(others[-1].view(torch.int64), take(other_src.view(torch.int64), order))))
-    del rs, others, order, sbits, radix, rk, seen, src
+    del rs, others, order, sbits, radix, rk, seen
+    out = dict(desc="stable sort_by_key, 10^8 16-byte records {u32 id, f32 score, u64 payload} keyed on the "
+                    "score (N(0,1), 1 % NaN/inf/-0.0, 20 % duplicated), records gathered; apply_order on u64[10^8]",
+               n=n, ms_per_step=dt * 1e3, records_per_s=n / dt, algo_bytes_per_record=120,
+               algorithmic_GBps=120 * n / dt / 1e9, roofline_frac=120 * n / dt / 1e9 / HBM_PEAK_GBS,
+               apply_order_ms=dt_apply * 1e3, apply_order_GBps=20 * n / dt_apply / 1e9,
+               apply_order_roofline_frac=20 * n / dt_apply / 1e9 / HBM_PEAK_GBS, ok=ok)
+    if host:
+        hr = torch.empty((n, 16), dtype=torch.uint8, pin_memory=True)
+        hr.copy_(src)
+        d = src.clone()
+        ho = torch.empty(n, dtype=torch.int32, pin_memory=True)
+
+        def fn():
+            ts.sort_by_key_with_buffer(d, buf, key)
+            ho.copy_(buf.order().view(torch.int32), non_blocking=True)
+
+        t = host_roundtrip([(hr, d)], fn, max(3, steps))
+        out["with_copies"] = dict(desc="records in pinned host memory: copied in, sorted and gathered, records and "
+                                       "order() copied out (wall time, median)", ms_per_step=t * 1e3,
+                                  records_per_s=n / t)
+        del hr, ho, d
+    del src, other_src
     torch.cuda.empty_cache()
-    return dict(desc="stable sort_by_key, 10^8 16-byte records {u32 id, f32 score, u64 payload} keyed on the "
-                     "score (N(0,1), 1 % NaN/inf/-0.0, 20 % duplicated), records gathered; apply_order on u64[10^8]",
-                n=n, ms_per_step=dt * 1e3, records_per_s=n / dt, algo_bytes_per_record=120,
-                algorithmic_GBps=120 * n / dt / 1e9, roofline_frac=120 * n / dt / 1e9 / HBM_PEAK_GBS,
-                apply_order_ms=dt_apply * 1e3, apply_order_GBps=20 * n / dt_apply / 1e9, ok=ok)
+    if cpu is not None:
+        from oracle import native as nv
 
-
-def run_host(steps: int, ctx, log2n: int = 28, seed: int = 2):
-    """sort_host on 2^28 u32 keys in pinned host memory: H2D + sort + D2H, the
-    end-to-end rate for a caller whose slice lives in host memory."""
-    import torch
-
-    import turbosort_amd as ts
-
-    n = 1 << log2n
-    dev = torch.device("cuda", 0)
-    d = torch.empty(n, dtype=torch.uint32, device=dev)
-    ts.fill_splitmix64(d, seed, ctx=ctx)
-    src = torch.empty(n, dtype=torch.uint32, pin_memory=True)
-    src.copy_(d)
-    before = ts.checksum(d, ctx=ctx)
-    h = torch.empty_like(src).pin_memory()
-    h.copy_(src)
-    ts.sort_host(h, ctx=ctx)  # warm: the context's device buffer grows once
-    times = []
-    for _ in range(steps):
-        h.copy_(src)
-        t0 = time.perf_counter()
-        ts.sort_host(h, ctx=ctx)
-        times.append(time.perf_counter() - t0)
-    d.copy_(h)
-    after = ts.checksum(d, ctx=ctx)
-    ok = after[0] == before[0] and after[1] == before[1] and after[2] == 0
-    t = sorted(times)[len(times) // 2]
-    del d, src, h
-    torch.cuda.empty_cache()
-    return dict(desc="sort_host: 2^28 u32 keys (SplitMix64 state 2) in pinned host memory; copy to the device, "
-                     "sort, copy back (wall time, median)", n=n, ms_per_step=t * 1e3, keys_per_s=n / t,
-                link_GBps_both_ways=2 * 4 * n / t / 1e9, ok=ok)
+        m = scores_sample.size
+        order_out = np.empty(m, dtype=np.uint32)
+        recs = np.zeros((m, 16), dtype=np.uint8)
+        t, runs = cpu_timed(lambda: None, lambda: (nv.lib().tso_sort_by_key_order(
+            8, scores_sample.ctypes.data, m, order_out.ctypes.data), nv.lib().tso_apply_order(
+            order_out.ctypes.data, m, recs.ctypes.data, m, 16)), cpu["budget"])
+        out["cpu_baseline"] = dict(value=m / t, unit="records/s", cores=1, kind="port",
+                                   sample=f"the first {m} scores of the same records: median of {runs} runs of the "
+                                          "restated order() of sort_by_key_with_buffer and move_into_order "
+                                          "(by_key.rs:175-235, 315-334; the reference's by-key path is serial)")
+    return out
 
 
 GAMMA = 0x9E3779B97F4A7C15
@@ -403,9 +580,11 @@ def equal_chunked(a, b, chunk: int = 1 << 26) -> bool:
     return all(bool(torch.equal(a[i:i + chunk], b[i:i + chunk])) for i in range(0, a.numel(), chunk))
 
 
-def run_learned(steps: int, ctx, n: int = 1_000_000_000, seed: int = 8):
+def run_learned(steps: int, ctx, n: int = 1_000_000_000, seed: int = 8, host: bool = True, cpu=None):
     """Learned partitioner against the plain device radix on 10^9 skewed u64
-    keys (and the same x as f64), with the untrained model as the ablation."""
+    keys (and the same x as f64), with the model fitted on a 1 % sample (the
+    workload's) and the output judged before the call returns; the 0.5 %
+    sample and the untrained model (the ablation) beside it."""
     import torch
 
     import turbosort_amd as ts
@@ -427,24 +606,57 @@ def run_learned(steps: int, ctx, n: int = 1_000_000_000, seed: int = 8):
             return dt, out
 
         dt_plain, ref = timed_copies(lambda b: ts.sort(b, ctx=ctx))
+        plain_info = ctx.last_sort_info()
+        cfg = ts.LearnedConfig(sample_per_mille=10)
         rep = {}
-        dt_learn, out = timed_copies(lambda b: rep.update(ts.learned_sort(b, ctx=ctx)))
-        ok = equal_chunked(out, ref)
+        dt_learn, out = timed_copies(lambda b: rep.update(ts.learned_sort(b, config=cfg, ctx=ctx)))
+        ok = equal_chunked(out, ref) and rep["verified"] == 1
+        del out
+        rep_h = {}
+        dt_half, out = timed_copies(lambda b: rep_h.update(ts.learned_sort(b, config=ts.LearnedConfig(sample_per_mille=5),
+                                                                           ctx=ctx)))
+        ok = ok and equal_chunked(out, ref)
+        del out
+        rep_nv = {}
+        dt_nv, out = timed_copies(lambda b: rep_nv.update(ts.learned_sort(
+            b, config=ts.LearnedConfig(sample_per_mille=10, verify=False), ctx=ctx)))
+        ok = ok and equal_chunked(out, ref)
         del out
         rep_u = {}
-        dt_untr, out = timed_copies(lambda b: rep_u.update(ts.learned_sort(b, config=ts.LearnedConfig(train=False),
-                                                                            ctx=ctx)))
+        dt_untr, out = timed_copies(lambda b: rep_u.update(ts.learned_sort(
+            b, config=ts.LearnedConfig(sample_per_mille=10, train=False), ctx=ctx)))
         ok = ok and equal_chunked(out, ref)
-        del out, ref, src
+        del out
+        r = dict(n=n, plain_ms=dt_plain * 1e3, learned_ms=dt_learn * 1e3, untrained_ms=dt_untr * 1e3,
+                 plain_keys_per_s=n / dt_plain, learned_keys_per_s=n / dt_learn,
+                 speedup_vs_plain=dt_plain / dt_learn, untrained_speedup_vs_plain=dt_plain / dt_untr,
+                 learned_roofline_frac=40 * n / dt_learn / 1e9 / HBM_PEAK_GBS,
+                 plain_roofline_frac=136 * n / dt_plain / 1e9 / HBM_PEAK_GBS,
+                 plain_executed_passes=plain_info["executed_passes"],
+                 overflow_rate=rep["overflow_keys"] / n, report=rep, untrained_report=rep_u,
+                 sample_half_percent=dict(learned_ms=dt_half * 1e3, speedup_vs_plain=dt_plain / dt_half,
+                                          overflow_rate=rep_h["overflow_keys"] / n, report=rep_h),
+                 without_judge=dict(learned_ms=dt_nv * 1e3, speedup_vs_plain=dt_plain / dt_nv), ok=ok)
+        if host and name == "u64":
+            hsrc = torch.empty(n, dtype=src.dtype, pin_memory=True)
+            hsrc.copy_(src)
+            d = ref  # device buffer of the same size
+            t = host_roundtrip([(hsrc, d)], lambda: ts.learned_sort(d, config=cfg, ctx=ctx), 2)
+            r["with_copies"] = dict(desc="keys in pinned host memory: copied in, learned sort (1 % sample, judged), "
+                                         "copied out (wall time, median)", ms_per_step=t * 1e3, keys_per_s=n / t)
+            del hsrc
+        if cpu is not None:
+            from oracle import closed_form as cf
+
+            sample = cf.lognormal_mixture_keys(seed, 1 << 25, as_f64=as_f64)
+            r["cpu_baseline"] = cpu_sort_baseline(sample, cpu["threads"], cpu["budget"],
+                                                  f"2^25 keys of the same mixture as {name}")
+        del ref, src
         torch.cuda.empty_cache()
-        res[name] = dict(n=n, plain_ms=dt_plain * 1e3, learned_ms=dt_learn * 1e3, untrained_ms=dt_untr * 1e3,
-                         plain_keys_per_s=n / dt_plain, learned_keys_per_s=n / dt_learn,
-                         speedup_vs_plain=dt_plain / dt_learn, untrained_speedup_vs_plain=dt_plain / dt_untr,
-                         learned_roofline_frac=40 * n / dt_learn / 1e9 / HBM_PEAK_GBS,
-                         plain_roofline_frac=136 * n / dt_plain / 1e9 / HBM_PEAK_GBS,
-                         overflow_rate=rep["overflow_keys"] / n, report=rep, untrained_report=rep_u, ok=ok)
+        res[name] = r
     return dict(desc="learned partitioner vs plain GPU radix, 10^9 keys: lognormal mixture (sigma 0.5/1/2) with "
-                     "5 % Zipf(1.1) hot values, as u64 and as f64; untrained (uniform-knot) model as the ablation",
+                     "5 % Zipf(1.1) hot values, as u64 and as f64; model trained on a 1 % sample, output judged "
+                     "(sortedness + multiset fingerprint); untrained (uniform-knot) model as the ablation",
                 ok=all(v["ok"] for v in res.values()), **res)
 
 
@@ -467,67 +679,44 @@ def pmc_traffic(workload: str, log2n: int):
     return None
 
 
-def cpu_baseline(budget_s: float):
-    """The C++ restatement of turbosort::sort_parallel on this host's cores."""
-    import numpy as np
-
-    from oracle import closed_form as cf
-    from oracle import native as nv
-
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(cores, 16))
-    n = 1 << 25
-    src = cf.u32_keys(2, n)
-    work = src.copy()
-    nv.sort_parallel_inplace(work, threads)  # warm-up
-    times = []
-    t_end = time.perf_counter() + budget_s
-    while time.perf_counter() < t_end or len(times) < 3:
-        np.copyto(work, src)
-        t0 = time.perf_counter()
-        nv.sort_parallel_inplace(work, threads)
-        times.append(time.perf_counter() - t0)
-        if len(times) >= 50:
-            break
-    times.sort()
-    med = times[len(times) // 2]
-    return dict(value=n / med, unit="keys/s", cores=threads, kind="port",
-                sample=f"2^25 u32 keys (SplitMix64 state 2), median of {len(times)} sorts by the C++ restatement "
-                       f"of turbosort::sort_parallel with {threads} threads")
-
-
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); every figure is per single MI355X.",
               file=sys.stderr)
         sys.exit(2)
-    import torch
+    import torch  # noqa: F401
 
     import turbosort_amd as ts
 
     ctx = ts.Context(0)
-    head = run_workload(a.workload, a.log2n, a.steps, a.warmup, ctx)
+    share = cpu_share()
+    cpu = None if a.no_cpu_baseline else dict(threads=share["threads"], budget=a.cpu_seconds)
+    host = not a.no_host
+    head = run_workload(a.workload, a.log2n, a.steps, a.warmup, ctx, host=host, cpu=cpu)
     also = {}
     for name in [x for x in a.also.split(",") if x and x != a.workload]:
         if name == "segmented":
-            r = run_segmented(max(3, a.steps // 2), ctx)
+            r = run_segmented(max(3, a.steps // 2), ctx, host=host, cpu=cpu)
         elif name == "by_key":
-            r = run_by_key(max(3, a.steps // 3), ctx)
+            r = run_by_key(max(3, a.steps // 3), ctx, host=host, cpu=cpu)
         elif name == "learned":
-            r = run_learned(max(2, a.steps // 5), ctx)
-        elif name == "host":
-            r = run_host(max(3, a.steps // 3), ctx)
+            r = run_learned(max(2, a.steps // 5), ctx, n=a.learned_n, host=host, cpu=cpu)
         else:
-            r = run_workload(name, a.log2n, max(3, a.steps // 3), 1, ctx)
-            r["roofline_frac"] = r["total_gbs"] / HBM_PEAK_GBS
-        also[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
-    cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_seconds)
+            r = run_workload(name, a.log2n, max(3, a.steps // 3), 1, ctx, host=host, cpu=cpu)
+        also[name] = r
+
+    def rounded(x):
+        if isinstance(x, float):
+            return round(x, 4)
+        if isinstance(x, dict):
+            return {k: rounded(v) for k, v in x.items()}
+        return x
 
     sweep_gbs = head["sweep_gbs"]
+    cpu_head = head.get("cpu_baseline")
+    if cpu_head is not None:
+        cpu_head = dict(cpu_head, host_cpus=share["nproc"], cpu_share=share["affinity"], cpu_model=share["model"])
     line = {
         "metric": "keys/s & % HBM roofline, 256M u32/u64; learned/segmented vs plain GPU radix",
         "value": head["keys_per_s"],
@@ -549,12 +738,14 @@ def main():
                      "kernel_ms": head["sweep_ms"]},
         "whole_sort": {"algorithmic_GBps": head["total_gbs"], "frac": head["total_gbs"] / HBM_PEAK_GBS,
                        "event_ms_median": head["event_ms_median"], "hist_ms": head["hist_ms"],
-                       "verified": head["ok"]},
-        "cpu_baseline": cpu,
-        "also": also,
+                       "executed": rounded(head["executed"]), "verified": head["ok"]},
+        "with_copies": rounded(head.get("with_copies")),
+        "cpu_baseline": rounded(cpu_head),
+        "also": {k: rounded(v) for k, v in also.items()},
     }
     print(json.dumps(line), flush=True)
-    if not head["ok"] or any(not v["ok"] for v in also.values()):
+    if not head["ok"] or any(not v["ok"] for v in also.values()) or (
+            head.get("with_copies") and not head["with_copies"]["ok"]):
         sys.exit(1)
 
 

@@ -41,6 +41,7 @@ def lib():
         L.tso_varying_bits.argtypes = [vp, sz, ctypes.c_int, vp]
         L.tso_finish_runs_u16.argtypes = [vp, sz, ctypes.c_uint32]
         L.tso_splitmix64_fill.argtypes = [ctypes.c_uint64, sz, vp]
+        L.tso_segmented_sort_pairs.argtypes = [ctypes.c_int, vp, vp, vp, sz, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -124,3 +125,13 @@ def splitmix64(seed: int, n: int) -> np.ndarray:
     out = np.empty(n, dtype=np.uint64)
     lib().tso_splitmix64_fill(seed, n, out.ctypes.data)
     return out
+
+
+def segmented_sort_pairs_inplace(keys: np.ndarray, offsets: np.ndarray, values=None, threads: int = 1) -> None:
+    """Every segment sorted stably by key with its u32 values, in place (the
+    restated per-segment sort_by_key_with_buffer + move)."""
+    assert keys.flags.c_contiguous
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    vp = None if values is None else values.ctypes.data
+    rc = lib().tso_segmented_sort_pairs(_code(keys), keys.ctypes.data, vp, off.ctypes.data, off.size - 1, threads)
+    assert rc == 0, rc

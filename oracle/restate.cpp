@@ -618,9 +618,39 @@ template <class T> struct OrderFn {
   }
 };
 
+// chunks_exact_mut(size).for_each(sort_by_key) over uneven slices
+// (benches/sort_bench.rs:229-250 pattern): per segment the by-key order
+// (by_key.rs:175-235), then keys and values moved into it (by_key.rs:315-334).
+// Segments are independent, so `threads` std::threads take them in turn.
+template <class T> struct SegFn {
+  static int run(void* keys, uint32_t* vals, const uint64_t* off, size_t nseg, int threads) {
+    T* k = static_cast<T*>(keys);
+    parallel_for(nseg, threads, [&](size_t s) {
+      const size_t a = size_t(off[s]), m = size_t(off[s + 1] - off[s]);
+      if (m < 2) return;
+      std::vector<uint32_t> order(m);
+      order_by_key<T>(k + a, m, order.data());
+      std::vector<T> kk(m);
+      std::vector<uint32_t> vv(m);
+      for (size_t i = 0; i < m; ++i) {
+        kk[i] = k[a + order[i]];
+        if (vals) vv[i] = vals[a + order[i]];
+      }
+      std::memcpy(k + a, kk.data(), m * sizeof(T));
+      if (vals) std::memcpy(vals + a, vv.data(), m * sizeof(uint32_t));
+    });
+    return TSO_OK;
+  }
+};
+
 }  // namespace
 
 extern "C" {
+
+int tso_segmented_sort_pairs(int dtype, void* keys, uint32_t* vals, const uint64_t* offsets, size_t n_segments,
+                             int threads) {
+  return by_dtype<SegFn>(dtype, keys, vals, offsets, n_segments, threads);
+}
 
 int tso_sort(int dtype, void* data, size_t n) { return by_dtype<SortFn>(dtype, data, n); }
 

@@ -49,6 +49,14 @@ int tso_sort_by_key_order(int key_dtype, const void* keys, size_t n, uint32_t* o
  * data[i] <- data[order[i]] for elements of elem_bytes bytes. */
 int tso_apply_order(const uint32_t* order, size_t order_len, void* data, size_t n, size_t elem_bytes);
 
+/* Every segment [offsets[s], offsets[s+1]) sorted stably by key with its
+ * u32 values (vals may be NULL): per segment the order of
+ * sort_by_key_with_buffer and the move into it (by_key.rs:175-235, 315-334),
+ * the reference's chunks-and-sort usage (benches/sort_bench.rs:229-250);
+ * `threads` std::threads take the segments in turn. */
+int tso_segmented_sort_pairs(int dtype, void* keys, uint32_t* vals, const uint64_t* offsets, size_t n_segments,
+                             int threads);
+
 /* Internal decisions, exported so the reference's own KATs can pin them. */
 uint32_t tso_log2_eighths(uint64_t x);                                       /* divert.rs:95-104 */
 size_t tso_skippable_digits(size_t n_live, size_t len, const uint64_t* max_bucket); /* divert.rs:38-54 */

@@ -311,3 +311,29 @@ def test_bench_generators_match_the_oracle_generators():
     nan = np.isnan(scores)
     assert 0.001 < nan.mean() < 0.01  # two of the five specials, 1 % of the records
     assert np.isinf(scores).any() and (rec["score"] == 0x80000000).any()
+
+
+def test_restated_segmented_sort_matches_the_closed_form():
+    """tso_segmented_sort_pairs (the CPU baseline of the segmented sort) gives
+    the closed form's per-segment stable order, with empty, one-key and long
+    segments and heavy ties, for several key types and thread counts."""
+    import numpy as np
+
+    from oracle import closed_form as cf
+    from oracle import native as nv
+
+    g = np.random.default_rng(11)
+    for name, threads in (("u32", 1), ("f32", 3), ("i64", 8), ("u8", 2)):
+        lens = g.integers(0, 600, size=300)
+        lens[:5] = [0, 1, 2, 0, 5000]
+        off = np.zeros(lens.size + 1, dtype=np.uint64)
+        off[1:] = np.cumsum(lens)
+        n = int(off[-1])
+        keys = cf.from_radix(g.integers(0, 40, size=n).astype(cf.UNSIGNED_OF[np.dtype(cf.NP_DTYPES[name]).itemsize]),
+                             name) if name != "f32" else g.standard_normal(n).astype(np.float32).round(1)
+        vals = np.arange(n, dtype=np.uint32)
+        k, v = keys.copy(), vals.copy()
+        nv.segmented_sort_pairs_inplace(k, off, v, threads)
+        wk, wv = cf.segmented_sort(keys, off, vals)
+        assert np.array_equal(k.view(cf.UNSIGNED_OF[k.dtype.itemsize]), wk.view(cf.UNSIGNED_OF[k.dtype.itemsize]))
+        assert np.array_equal(v, wv)
