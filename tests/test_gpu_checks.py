@@ -1,9 +1,10 @@
 """The device-side checks, and configurations no other GPU test covers.
 
 * ts_checksum, the fingerprint behind every full-size "sorted, same keys"
-  claim: equal to a numpy restatement (sum and xor of the SplitMix64 mixer of
-  each key's bits, and the count of adjacent pairs out of order in the type's
-  total order); one swapped pair or one changed key changes it.
+  claim: equal to a numpy restatement (sum and xor of a one-multiply
+  xor-shift mixer of each key's salted bits, and the count of adjacent pairs
+  out of order in the type's total order); one swapped pair or one changed
+  key changes it.
 * The sticky device error word (ts_ctx_check): a segment longer than the
   max_segment_len given to an asynchronous segmented sort, and invalid
   offsets in such a call, are reported by the next check and then cleared.
@@ -24,22 +25,22 @@ from shapes import random_of, rng
 
 pytestmark = pytest.mark.gpu
 
-M1 = np.uint64(0xBF58476D1CE4E5B9)
-M2 = np.uint64(0x94D049BB133111EB)
 SALT = np.uint64(0x632BE59BD9B4E019)
+MUL = np.uint64(0xD6E8FEB86659FD93)
 
 
-def mix64(z: np.ndarray) -> np.ndarray:
+def mix(z: np.ndarray) -> np.ndarray:
+    """The fingerprint's mixer: xor-shift 32, multiply, xor-shift 32 (mod 2^64)."""
     with np.errstate(over="ignore"):
-        z = (z ^ (z >> np.uint64(30))) * M1
-        z = (z ^ (z >> np.uint64(27))) * M2
-        return z ^ (z >> np.uint64(31))
+        z = z ^ (z >> np.uint64(32))
+        z = z * MUL
+        return z ^ (z >> np.uint64(32))
 
 
 def fingerprint(x: np.ndarray):
-    """Sum and xor (mod 2^64) of mix64(bits + salt), and the unsorted pairs."""
+    """Sum and xor (mod 2^64) of mix(bits + salt), and the unsorted pairs."""
     with np.errstate(over="ignore"):
-        h = mix64(bits(x).astype(np.uint64) + SALT)
+        h = mix(bits(x).astype(np.uint64) + SALT)
     s = int(np.sum(h, dtype=np.uint64))
     xr = int(np.bitwise_xor.reduce(h)) if h.size else 0
     k = cf.to_radix(x)

@@ -52,11 +52,23 @@ template <class U> __host__ __device__ __forceinline__ U from_key(U key, int mod
   return key;
 }
 
-// A 64-bit finaliser (the SplitMix64 mixer) used by the multiset checksum.
+// A 64-bit finaliser (the SplitMix64 mixer): the synthetic key streams and the
+// learned sample's strata.
 __host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
+}
+
+// One key's share of the multiset fingerprint: its bits (zero-extended) plus
+// a salt, through a one-multiply xor-shift mixer (cheap enough for the
+// passes that take it on the side).  A fingerprint is the sum and the xor of
+// these over the keys, which every permutation keeps.
+__host__ __device__ __forceinline__ uint64_t fingerprint_of(uint64_t bits) {
+  uint64_t x = bits + 0x632BE59BD9B4E019ull;
+  x ^= x >> 32;
+  x *= 0xD6E8FEB86659FD93ull;
+  return x ^ (x >> 32);
 }
 
 __host__ __device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {

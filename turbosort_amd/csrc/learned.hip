@@ -525,6 +525,7 @@ template <class U> hipError_t run_learned(const LearnedJob& j, hipStream_t s, Ke
   p.num_cus = j.num_cus;
   p.atomic_rank = j.atomic_rank;
   p.model = j.model;
+  p.fingerprint = j.fingerprint;
   e = radix_sort(p, s, clk);
   if (e != hipSuccess) return e;
   if (size_t(j.buckets) * kBoundSearchRatio <= j.n) {

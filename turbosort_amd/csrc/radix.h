@@ -150,6 +150,10 @@ struct RadixJob {
   // Rank keys within a wave by LDS atomic adds instead of ballots; only where
   // rank_order_check found the adds applied in lane order.
   bool atomic_rank = false;
+  // Learned partition: the histogram read also adds the keys' multiset
+  // fingerprint (sum, xor of fingerprint_of) into fingerprint[0..1], which
+  // the caller has zeroed (null: not taken).
+  unsigned long long* fingerprint = nullptr;
 };
 
 hipError_t radix_sort(const RadixJob& job, hipStream_t s, KernelClock* clk);
@@ -275,6 +279,7 @@ struct LearnedJob {
   uint32_t* error = nullptr;  // as RadixJob::error
   int num_cus = 256;
   bool atomic_rank = false;  // as RadixJob::atomic_rank
+  unsigned long long* fingerprint = nullptr;  // as RadixJob::fingerprint, for the partition's histogram
 };
 hipError_t learned_partition(const LearnedJob& job, hipStream_t s, KernelClock* clk);
 
@@ -288,5 +293,7 @@ hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStre
 // sum and xor of mix64(bits + salt), and the adjacent pairs out of order.
 hipError_t checksum(int key_bytes, int mode, const void* data, size_t n, unsigned long long* out, hipStream_t s,
                     KernelClock* clk = nullptr);
+// Zeroes the three words of a fingerprint slot (a kernel, not a memset).
+hipError_t fingerprint_zero(unsigned long long* out, hipStream_t s);
 
 }  // namespace ts

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
                                                        size_t n, int mode, size_t seg0, uint32_t blocks_per_seg,
                                                        size_t per_block, SortMeta* m, uint32_t* zero_ptr,
                                                        size_t zero_words, const LearnedModel* model,
-                                                       int check_order, int gate) {
+                                                       int check_order, int gate, unsigned long long* fp) {
   using D = typename DigitSource<U, LEARNED>::T;
   constexpr int P = DigitSource<U, LEARNED>::P;
   constexpr int UN = HistUnroll<U>::value;
@@ -209,6 +209,7 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
   const size_t seg_lo = min(n, size_t(s0) * seg0), seg_hi = min(n, seg_lo + seg0);
   const size_t begin = min(seg_hi, seg_lo + size_t(blockIdx.x % blocks_per_seg) * per_block);
   const size_t end = min(seg_hi, begin + per_block);
+  unsigned long long fp_sum = 0, fp_xor = 0;
   for (size_t base = begin; base < end; base += size_t(kHistThreads) * UN) {
     const bool full = base + size_t(kHistThreads) * UN <= end;
     U k[UN];
@@ -217,6 +218,17 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
       size_t i = base + size_t(r) * kHistThreads + tid;
       if (!full) i = i < end ? i : end - 1;
       k[r] = REC ? *reinterpret_cast<const U*>(rec + i * stride + key_off) : keys[i];
+    }
+    if (LEARNED && fp) {
+      // The input's multiset fingerprint for the learned sort's judge.
+#pragma unroll
+      for (int r = 0; r < UN; ++r) {
+        if (full || base + size_t(r) * kHistThreads + tid < end) {
+          const uint64_t hh = fingerprint_of(uint64_t(k[r]));
+          fp_sum += hh;
+          fp_xor ^= hh;
+        }
+      }
     }
     to_keys(k, mode);
     if (REC) {
@@ -257,6 +269,37 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
       if (__ballot((diff & care) != 0) == 0) {
         const uint32_t seg = p == 0 ? s0 : uint32_t(uint64_t(k0) >> (8 * p - 3)) & 7u;
         if (lane == 0) atomicAdd(&hp[seg * 256 + (uint32_t(uint64_t(k0) >> (8 * p)) & 0xFFu)], wave_count);
+      } else if (p > 0 && __ballot((diff & (0xFFull << (8 * p))) != 0) == 0) {
+        // The digit is the same in every key of the step but the segment
+        // bits differ (a dead digit above a live one): count the segments in
+        // 16-bit fields of two registers instead of 64 colliding LDS adds
+        // onto at most eight counters (at most 64 * UN < 2^16 per field).
+        uint64_t lo4 = 0, hi4 = 0;
+#pragma unroll
+        for (int r = 0; r < UN; ++r) {
+          const size_t i = base + size_t(r) * kHistThreads + tid;
+          if (full || i < end) {
+            const uint32_t seg = uint32_t(uint64_t(dk[r]) >> (8 * p - 3)) & 7u;
+            const uint64_t one = 1ull << (16 * (seg & 3u));
+            if (seg < 4)
+              lo4 += one;
+            else
+              hi4 += one;
+          }
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+          lo4 += __shfl_xor(lo4, d, 64);
+          hi4 += __shfl_xor(hi4, d, 64);
+        }
+        if (lane == 0) {
+          const uint32_t dig = uint32_t(uint64_t(k0) >> (8 * p)) & 0xFFu;
+#pragma unroll
+          for (int sg = 0; sg < kSegments; ++sg) {
+            const uint32_t c = uint32_t(((sg < 4 ? lo4 : hi4) >> (16 * (sg & 3))) & 0xFFFFu);
+            if (c) atomicAdd(&hp[sg * 256 + dig], c);
+          }
+        }
       } else {
 #pragma unroll
         for (int r = 0; r < UN; ++r) {
@@ -267,6 +310,17 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
           }
         }
       }
+    }
+  }
+  if (LEARNED && fp) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      fp_sum += __shfl_xor(fp_sum, d, 64);
+      fp_xor ^= __shfl_xor(fp_xor, d, 64);
+    }
+    if (lane == 0) {
+      atomicAdd(&fp[0], fp_sum);
+      atomicXor(&fp[1], fp_xor);
     }
   }
   __syncthreads();
@@ -1166,15 +1220,15 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
     if constexpr (LEARNED)
       hipLaunchKernelGGL((k_hist<U, false, true>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
                          nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                         static_cast<uint32_t*>(j.status), zero_words, j.model, co, gate);
+                         static_cast<uint32_t*>(j.status), zero_words, j.model, co, gate, j.fingerprint);
     else if (j.records)
       hipLaunchKernelGGL((k_hist<U, true>), hgrid, dim3(kHistThreads), 0, s, nullptr, j.records, j.stride,
                          j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                         static_cast<uint32_t*>(j.status), zero_words, nullptr, co, gate);
+                         static_cast<uint32_t*>(j.status), zero_words, nullptr, co, gate, nullptr);
     else
       hipLaunchKernelGGL((k_hist<U, false>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
                          nullptr, size_t(0), size_t(0), nullptr, j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
-                         static_cast<uint32_t*>(j.status), zero_words, nullptr, co, gate);
+                         static_cast<uint32_t*>(j.status), zero_words, nullptr, co, gate, nullptr);
     if (clk) clk->end(s);
   };
   hist(0);

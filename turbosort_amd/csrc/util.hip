@@ -62,18 +62,42 @@ __global__ void k_check_zero(unsigned long long* out) {
   if (threadIdx.x < 3) out[threadIdx.x] = 0;
 }
 
-// Sum and xor of mix64(bits) over the keys (a multiset fingerprint that any
-// permutation preserves) and the number of adjacent pairs out of order.
+// Sum and xor of fingerprint_of(bits) over the keys (a multiset fingerprint
+// that any permutation keeps) and the number of adjacent pairs out of order.
+// Each lane takes kCheckPer keys of a wave's 64 * kCheckPer-key stretch per
+// step, all loads in flight together; a key's successor comes from the next
+// lane (a shuffle), and only the wave's last lane loads the key after the
+// stretch.
+constexpr int kCheckPer = 8;
+
 template <class U>
-__global__ void k_check(const U* __restrict__ x, size_t n, int mode, unsigned long long* out) {
-  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+__global__ __launch_bounds__(kThreads) void k_check(const U* __restrict__ x, size_t n, int mode,
+                                                    unsigned long long* out) {
+  const unsigned lane = threadIdx.x & 63;
+  const size_t wave = (size_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const size_t waves = (size_t(gridDim.x) * kThreads) >> 6;
+  constexpr size_t kStretch = size_t(64) * kCheckPer;
   unsigned long long sum = 0, xr = 0, bad = 0;
-  for (size_t i = g; i < n; i += gs) {
-    const U a = x[i];
-    const uint64_t h = mix64(uint64_t(a) + 0x632BE59BD9B4E019ull);
-    sum += h;
-    xr ^= h;
-    if (i + 1 < n && to_key(a, mode) > to_key(x[i + 1], mode)) ++bad;
+  for (size_t b = wave * kStretch; b < n; b += waves * kStretch) {
+    U a[kCheckPer];
+#pragma unroll
+    for (int r = 0; r < kCheckPer; ++r) a[r] = x[min(b + size_t(r) * 64 + lane, n - 1)];
+    const size_t after = min(b + kStretch, n - 1);
+    const U tail = x[after];  // the key after the stretch (lane 63 uses it)
+#pragma unroll
+    for (int r = 0; r < kCheckPer; ++r) {
+      const size_t i = b + size_t(r) * 64 + lane;
+      // The next key: the next lane's, or the next row's first (lane 0).
+      U nx = __shfl_down(a[r], 1, 64);
+      const U first_next = r + 1 < kCheckPer ? __shfl(a[r + 1 < kCheckPer ? r + 1 : r], 0, 64) : tail;
+      if (lane == 63) nx = first_next;
+      if (i < n) {
+        const uint64_t h = fingerprint_of(uint64_t(a[r]));
+        sum += h;
+        xr ^= h;
+        if (i + 1 < n && to_key(a[r], mode) > to_key(nx, mode)) ++bad;
+      }
+    }
   }
   for (int d = 32; d >= 1; d >>= 1) {
     sum += __shfl_xor(sum, d, 64);
@@ -84,7 +108,7 @@ __global__ void k_check(const U* __restrict__ x, size_t n, int mode, unsigned lo
   // so the three global words see little contention.
   __shared__ unsigned long long part[3][kThreads / 64];
   const unsigned w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
+  if (lane == 0) {
     part[0][w] = sum;
     part[1][w] = xr;
     part[2][w] = bad;
@@ -138,6 +162,11 @@ hipError_t gather(const uint32_t* order, const void* src, void* dst, size_t n, s
 hipError_t fill_splitmix64(void* out, size_t n, uint64_t seed, int kind, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_splitmix, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, out, n, seed, kind);
+  return hipGetLastError();
+}
+
+hipError_t fingerprint_zero(unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_check_zero, dim3(1), dim3(64), 0, s, out);
   return hipGetLastError();
 }
 
