@@ -693,7 +693,9 @@ def main():
     share = cpu_share()
     cpu = None if a.no_cpu_baseline else dict(threads=share["threads"], budget=a.cpu_seconds)
     host = not a.no_host
+    t0 = time.perf_counter()
     head = run_workload(a.workload, a.log2n, a.steps, a.warmup, ctx, host=host, cpu=cpu)
+    print(f"bench: {a.workload} done ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     also = {}
     for name in [x for x in a.also.split(",") if x and x != a.workload]:
         if name == "segmented":
@@ -705,6 +707,7 @@ def main():
         else:
             r = run_workload(name, a.log2n, max(3, a.steps // 3), 1, ctx, host=host, cpu=cpu)
         also[name] = r
+        print(f"bench: {name} done ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
 
     def rounded(x):
         if isinstance(x, float):
