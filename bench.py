@@ -461,6 +461,12 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5, host: bool 
     for i in range(warm):
         ts.sort_by_key_with_buffer(rs[i], buf, key)
     dt = timed(lambda i: ts.sort_by_key_with_buffer(rs[warm + i], buf, key), steps)
+    passes = int(buf._ctx.last_sort_info()["executed_passes"])
+    # The 16-byte records move through the passes with their u32 positions:
+    # the histogram reads each record once (16 B), the first pass reads it and
+    # writes record + position (36 B), every later pass reads and writes both
+    # (40 B; the last writes the position into order()).
+    executed = 16 + (36 + 40 * (passes - 1) if passes else 0)
     others = [other_src.clone() for _ in range(steps)]
     dt_apply = timed(lambda i: buf.apply_order(others[i]), steps)
     # order() is the stable argsort iff it is a permutation along which the
@@ -477,9 +483,12 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5, host: bool 
           and bool(torch.equal(others[-1].view(torch.int64), take(other_src.view(torch.int64), order))))
     del rs, others, order, sbits, radix, rk, seen
     out = dict(desc="stable sort_by_key, 10^8 16-byte records {u32 id, f32 score, u64 payload} keyed on the "
-                    "score (N(0,1), 1 % NaN/inf/-0.0, 20 % duplicated), records gathered; apply_order on u64[10^8]",
+                    "score (N(0,1), 1 % NaN/inf/-0.0, 20 % duplicated), records carried through the key passes; "
+                    "apply_order on u64[10^8]",
                n=n, ms_per_step=dt * 1e3, records_per_s=n / dt, algo_bytes_per_record=120,
                algorithmic_GBps=120 * n / dt / 1e9, roofline_frac=120 * n / dt / 1e9 / HBM_PEAK_GBS,
+               executed=dict(passes=passes, bytes_per_record=executed,
+                             roofline_frac=executed * n / dt / 1e9 / HBM_PEAK_GBS),
                apply_order_ms=dt_apply * 1e3, apply_order_GBps=20 * n / dt_apply / 1e9,
                apply_order_roofline_frac=20 * n / dt_apply / 1e9 / HBM_PEAK_GBS, ok=ok)
     if host:
@@ -493,8 +502,8 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5, host: bool 
             ho.copy_(buf.order().view(torch.int32), non_blocking=True)
 
         t = host_roundtrip([(hr, d)], fn, max(3, steps))
-        out["with_copies"] = dict(desc="records in pinned host memory: copied in, sorted and gathered, records and "
-                                       "order() copied out (wall time, median)", ms_per_step=t * 1e3,
+        out["with_copies"] = dict(desc="records in pinned host memory: copied in, sorted, records and order() "
+                                       "copied out (wall time, median)", ms_per_step=t * 1e3,
                                   records_per_s=n / t)
         del hr, ho, d
     del src, other_src

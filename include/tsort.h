@@ -25,7 +25,7 @@ typedef enum ts_status {
   TS_OK = 0,
   TS_ERR_BUFFER_TOO_SMALL = 1, /* dispatch.rs:78-83, radix/mod.rs:92-97 (panic) */
   TS_ERR_LENGTH_MISMATCH = 2,  /* by_key.rs:137-141 (panic) */
-  TS_ERR_TOO_LONG = 3,         /* by_key.rs:190-194: order() stays empty above u32::MAX */
+  TS_ERR_TOO_LONG = 3,         /* more than u32::MAX elements where an order was asked for (by_key.rs:190-194) */
   TS_ERR_INVALID_ARG = 4,
   TS_ERR_HIP = 5,              /* a HIP runtime call failed; see ts_ctx_last_error */
   TS_ERR_VERIFY_FAILED = 6,    /* an output failed sortedness or the multiset checksum */
@@ -90,12 +90,30 @@ ts_status ts_sort_host(ts_ctx* ctx, ts_dtype dtype, void* host_data, size_t n, v
  * POD records: record i starts at d_records + i*stride and its key, of type
  * key_type, at byte key_offset.  Writes d_order_out[i] = original position of
  * the record that sorts to i (SortByKeyBuffer::order, by_key.rs:108-110), the
- * stable order.  If gather_records != 0 the records are then moved into that
- * order in place (move_into_order, by_key.rs:315-334).  n > u32::MAX is
- * TS_ERR_TOO_LONG.  stride == key width and key_offset == 0 sorts a plain key
- * array by itself and yields its stable argsort. */
+ * stable order.  If gather_records != 0 the records are also moved into that
+ * order in place (move_into_order, by_key.rs:315-334); 8- and 16-byte records
+ * aligned to their size move through the sort's passes instead of being
+ * gathered afterwards.  stride == key width and key_offset == 0 sorts a plain
+ * key array by itself and yields its stable argsort.
+ * n > u32::MAX (by_key.rs:190-194): no order is written (d_order_out is not
+ * used and may be NULL) and the records are sorted stably in place, as
+ * ts_sort_by_key_unordered does; with gather_records == 0 that is
+ * TS_ERR_TOO_LONG. */
 ts_status ts_sort_by_key(ts_ctx* ctx, ts_dtype key_type, void* d_records, size_t n, size_t stride,
                          size_t key_offset, uint32_t* d_order_out, int gather_records, void* stream);
+
+/* The by-key sort beyond u32::MAX elements (by_key.rs:190-194: order()
+ * cleared, the slice sorted stably by key), for any n: the n records of
+ * record_bytes bytes at d_records are put in stable order of the keys of
+ * type key_type at d_keys + i*key_stride + key_offset, which may lie inside
+ * the records (d_keys == d_records, key_stride == record_bytes) or in an
+ * array of their own (the keys a key closure gave, evaluated once).  No order
+ * is produced.  The device memory it takes grows with n: 16-byte (key,
+ * position) records twice over plus one copy of the records, less when the
+ * records are the keys themselves or 8- or 16-byte records that hold them. */
+ts_status ts_sort_by_key_unordered(ts_ctx* ctx, ts_dtype key_type, const void* d_keys, size_t key_stride,
+                                   size_t key_offset, void* d_records, size_t n, size_t record_bytes,
+                                   void* stream);
 
 /* SortByKeyBuffer::apply_order (by_key.rs:136-144): d_other[i] <- d_other[order[i]]
  * for n elements of elem_bytes bytes, in place.  order_len != n is
@@ -297,8 +315,10 @@ ts_status ts_ctx_last_sort_info(ts_ctx* ctx, ts_sort_info* info, void* stream);
 /* Workspace planning, host arithmetic only (touches no device): bytes of
  * look-back status one radix sort of n keys of `dtype` uses with scatter
  * tiles of `kind` (0 keys only, 1 keys with u32 order values, 2 learned
- * bucket digits), or, for kind -1, the bytes a context reserves so that every
- * radix sort of at most n keys of that width fits, whatever its kind. */
+ * bucket digits, 3 / 4 carried 8- / 16-byte records with the order, 5 / 6
+ * carried 8- / 16-byte records without it), or, for kind -1, the bytes a
+ * context reserves so that every radix sort of at most n keys of that width
+ * and of the kinds 0-2 fits. */
 size_t ts_plan_status_bytes(ts_dtype dtype, size_t n, int kind);
 
 #ifdef __cplusplus

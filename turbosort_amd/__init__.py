@@ -282,11 +282,14 @@ class SortByKeyBuffer:
         b._ctx.reserve(key_dtype, length, record_bytes)
         return b
 
-    def _ensure(self, device: torch.device, n: int) -> None:
+    def _ensure_context(self, device: torch.device) -> None:
         if self._ctx is None or self._ctx.device.index != (device.index if device.index is not None else
                                                            torch.cuda.current_device()):
             self._ctx = Context(device)
             self._order = None
+
+    def _ensure(self, device: torch.device, n: int) -> None:
+        self._ensure_context(device)
         if self._order is None or self._order.numel() < n:
             self._order = torch.empty(max(n, 1), dtype=torch.uint32, device=self._ctx.device)
 
@@ -315,7 +318,8 @@ class SortByKeyBuffer:
                 raise TurbosortPanic("apply_order: the slice's length differs from the last sorted slice's")
             return
         ctx = self._ctx
-        ctx.check(ctx._lib.ts_apply_order(ctx.handle, ctypes.c_void_p(self._order.data_ptr()), self._len,
+        order = ctypes.c_void_p(self._order.data_ptr() if self._order is not None else None)
+        ctx.check(ctx._lib.ts_apply_order(ctx.handle, order, self._len,
                                           ctypes.c_void_p(other.data_ptr()), n, _record_bytes(other) if n else 1,
                                           _stream(other.device)))
 
@@ -332,8 +336,9 @@ def sort_by_key_with_buffer(records: torch.Tensor, buffer: SortByKeyBuffer, key:
     """
     _slice(records, "records")
     n = records.shape[0] if records.dim() > 0 else 0
-    if n > 0xFFFFFFFF:
-        raise NotImplementedError("more than u32::MAX elements: the order does not fit u32")
+    if n > U32_MAX:
+        _sort_by_key_unordered(records, buffer, key, n)
+        return
     buffer._ensure(records.device, n)
     ctx = buffer._ctx
     stream = _stream(records.device)
@@ -356,6 +361,35 @@ def sort_by_key_with_buffer(records: torch.Tensor, buffer: SortByKeyBuffer, key:
             ctx.check(ctx._lib.ts_apply_order(ctx.handle, order_ptr, n, ctypes.c_void_p(records.data_ptr()), n,
                                               _record_bytes(records), stream))
     buffer._len = n
+
+
+U32_MAX = 0xFFFFFFFF
+
+
+def _sort_by_key_unordered(records: torch.Tensor, buffer: SortByKeyBuffer, key: KeyLike, n: int) -> None:
+    """More elements than a u32 order can index (by_key.rs:190-194): the
+    records are sorted stably by the key, and the buffer's order is left
+    empty, so that a later apply_order of a non-empty slice is a length
+    mismatch."""
+    buffer._ensure_context(records.device)
+    ctx = buffer._ctx
+    stream = _stream(records.device)
+    if isinstance(key, KeyField):
+        code = _dtype_code(key.dtype)
+        rb = _record_bytes(records)
+        rc = ctx._lib.ts_sort_by_key_unordered(ctx.handle, code, ctypes.c_void_p(records.data_ptr()), rb, key.offset,
+                                               ctypes.c_void_p(records.data_ptr()), n, rb, stream)
+    else:
+        keys = records if key is None else key(records)
+        _slice(keys, "key tensor")
+        if keys.dim() != 1 or keys.shape[0] != n:
+            raise ValueError("the key must give one value per element")
+        code = _dtype_code(keys.dtype)
+        rc = ctx._lib.ts_sort_by_key_unordered(ctx.handle, code, ctypes.c_void_p(keys.data_ptr()), keys.element_size(),
+                                               0, ctypes.c_void_p(records.data_ptr()), n, _record_bytes(records),
+                                               stream)
+    ctx.check(rc)
+    buffer._len = 0
 
 
 def sort_by_key(records: torch.Tensor, key: KeyLike = None) -> None:

@@ -22,7 +22,7 @@ EXPORTS = [
     "ts_segmented_variants", "ts_segmented_variant_name", "ts_segmented_set_variant",
     "ts_ctx_set_ranking", "ts_ctx_ranking", "ts_sort_host", "ts_ctx_set_split_rounds", "ts_ctx_split_rounds",
     "ts_ctx_set_divert_min", "ts_ctx_divert_min", "ts_ctx_check", "ts_ctx_last_sort_info", "ts_plan_status_bytes",
-    "ts_verify_sorted", "ts_sort_with_buffer_static",
+    "ts_verify_sorted", "ts_sort_with_buffer_static", "ts_sort_by_key_unordered",
 ]
 
 
@@ -90,6 +90,8 @@ def load() -> ctypes.CDLL:
     L.ts_sort_by_key.restype = i32
     L.ts_apply_order.argtypes = [vp, vp, sz, vp, sz, sz, vp]
     L.ts_apply_order.restype = i32
+    L.ts_sort_by_key_unordered.argtypes = [vp, i32, vp, sz, sz, vp, sz, sz, vp]
+    L.ts_sort_by_key_unordered.restype = i32
     L.ts_segmented_sort_pairs.argtypes = [vp, i32, vp, vp, sz, vp, sz, sz, vp]
     L.ts_segmented_sort_pairs.restype = i32
     L.ts_segmented_lds_max_len.argtypes = [i32, i32]

@@ -665,6 +665,10 @@ size_t ts_plan_status_bytes(ts_dtype dt, size_t n, int kind) {
     case ts::kSweepKeys: return ts::status_bytes_used(n, kb, false, false);
     case ts::kSweepVals: return ts::status_bytes_used(n, kb, false, true);
     case ts::kSweepLearned: return kb >= 4 ? ts::status_bytes_used(n, kb, true, false) : 0;
+    case ts::kSweepRec8Vals: return ts::status_bytes_used(n, kb, false, true, 2);
+    case ts::kSweepRec16Vals: return ts::status_bytes_used(n, kb, false, true, 4);
+    case ts::kSweepRec8: return ts::status_bytes_used(n, kb, false, false, 2);
+    case ts::kSweepRec16: return ts::status_bytes_used(n, kb, false, false, 4);
     default: return 0;
   }
 }
@@ -787,6 +791,86 @@ ts_status ts_sort_host(ts_ctx* c, ts_dtype dt, void* host_data, size_t n, void* 
   return check_flags(c, s);
 }
 
+namespace {
+// By-key sorts of 8- and 16-byte records (aligned to their size) carry the
+// records through the radix passes instead of gathering them afterwards: a
+// random gather reads a whole 128-byte line per record, the passes read and
+// write them in runs.
+bool carries(const void* records, size_t n, size_t stride) {
+  return n > ts::kSmallMax && (stride == 8 || stride == 16) && reinterpret_cast<uintptr_t>(records) % stride == 0;
+}
+size_t carry_ws(size_t n, int kb, size_t stride, bool order) {
+  return up(sizeof(ts::SortMeta)) + up(ts::status_bytes_used(n, kb, false, order, int(stride / 4))) +
+         up(n * stride) + (order ? 2 * up(n * 4) : 0);
+}
+// The carried sort with its workspace at p (carry_ws bytes): records end in
+// stable key order in place, and the order is written when `order` is set.
+ts_status sort_carried(ts_ctx* c, int kb, int mode, void* records, size_t n, size_t stride, size_t key_offset,
+                       uint32_t* order, char* p, hipStream_t s) {
+  const int rw = int(stride / 4);
+  const size_t sb = ts::status_bytes_used(n, kb, false, order != nullptr, rw);
+  ts::RadixJob j = radix_job(c, kb, mode, n, p, p + up(sizeof(ts::SortMeta)), sb);
+  p += up(sizeof(ts::SortMeta)) + up(sb);
+  j.records = static_cast<const unsigned char*>(records);
+  j.stride = stride;
+  j.key_offset = key_offset;
+  j.rec_words = rw;
+  j.recs_b = p;
+  p += up(n * stride);
+  if (order) {
+    j.vals_a = reinterpret_cast<uint32_t*>(p);
+    p += up(n * 4);
+    j.vals_b = reinterpret_cast<uint32_t*>(p);
+    j.final_vals = order;
+  }
+  hipError_t e = ts::radix_sort(j, s, &c->clk);
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "radix_sort (records carried)");
+}
+
+// A stable sort of n records by keys read at keys + i * key_stride +
+// key_offset (inside the records or beside them), with no order: the by-key
+// sort of slices longer than a u32 order can index (by_key.rs:190-194).
+// Bare keys sort by themselves (equal keys are equal bits); 8- and 16-byte
+// records carrying their keys are carried; anything else sorts 16-byte
+// (key, u64 position) records and gathers by their positions.
+ts_status sort_unordered(ts_ctx* c, ts_dtype kt, const void* keys, size_t key_stride, size_t key_offset, void* records,
+                         size_t n, size_t record_bytes, hipStream_t s) {
+  const int kb = ts::dtype_bytes(kt), mode = ts::dtype_mode(kt);
+  const bool inside = keys == records && key_stride == record_bytes;
+  if (inside && record_bytes == size_t(kb)) return sort_impl(c, kt, records, n, nullptr, s);
+  if (n <= ts::kSmallMax) {  // one workgroup's (key, position) sort, then the gather
+    ts_status st = ensure(c, up(n * 4) + up(n * record_bytes));
+    if (st != TS_OK) return st;
+    uint32_t* order = reinterpret_cast<uint32_t*>(c->ws);
+    char* gbuf = c->ws + up(n * 4);
+    hipError_t e = ts::small_sort(kb, mode, nullptr, n, static_cast<const unsigned char*>(keys), key_stride,
+                                  key_offset, order, s, &c->clk);
+    if (e == hipSuccess) e = ts::gather(order, records, gbuf, n, record_bytes, s, &c->clk);
+    if (e == hipSuccess) e = hipMemcpyAsync(records, gbuf, n * record_bytes, hipMemcpyDeviceToDevice, s);
+    return e == hipSuccess ? TS_OK : hip_fail(c, e, "small by-key sort");
+  }
+  if (inside && carries(records, n, record_bytes)) {
+    ts_status st = ensure(c, carry_ws(n, kb, record_bytes, false));
+    if (st != TS_OK) return st;
+    return sort_carried(c, kb, mode, records, n, record_bytes, key_offset, nullptr, c->ws, s);
+  }
+  const size_t sort_bytes = carry_ws(n, kb, 16, false);
+  ts_status st = ensure(c, sort_bytes + up(n * 16) + up(n * record_bytes));
+  if (st != TS_OK) return st;
+  char* packed = c->ws + sort_bytes;
+  char* gbuf = packed + up(n * 16);
+  hipError_t e = ts::pack_key_pos(kb, keys, key_stride, key_offset, packed, n, s);
+  if (e != hipSuccess) return hip_fail(c, e, "pack_key_pos");
+  st = sort_carried(c, kb, mode, packed, n, 16, 0, nullptr, c->ws, s);
+  if (st != TS_OK) return st;
+  e = ts::gather_packed(packed, records, gbuf, n, record_bytes, s, &c->clk);
+  if (e != hipSuccess) return hip_fail(c, e, "gather");
+  e = hipMemcpyAsync(records, gbuf, n * record_bytes, hipMemcpyDeviceToDevice, s);
+  return e == hipSuccess ? TS_OK : hip_fail(c, e, "hipMemcpyAsync");
+}
+
+}  // namespace
+
 ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t stride, size_t key_offset,
                          uint32_t* order, int gather_records, void* stream) {
   if (!c) return TS_ERR_INVALID_ARG;
@@ -796,13 +880,24 @@ ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t
     return fail(c, TS_ERR_INVALID_ARG, "key does not fit inside the record stride");
   if ((reinterpret_cast<uintptr_t>(records) + key_offset) % size_t(kb) != 0 || stride % size_t(kb) != 0)
     return fail(c, TS_ERR_INVALID_ARG, "record keys must be aligned to the key width");
-  if (n > size_t(UINT32_MAX))  // by_key.rs:190-194
-    return fail(c, TS_ERR_TOO_LONG, "more than u32::MAX elements: order() stays empty");
+  if (n > size_t(UINT32_MAX)) {  // by_key.rs:190-194: a stable sort, and no order
+    if (!gather_records)
+      return fail(c, TS_ERR_TOO_LONG, "more than u32::MAX elements: no order, so only a sort that moves the records");
+    if (!records) return fail(c, TS_ERR_INVALID_ARG, "null records");
+    DeviceScope ds(c->device);
+    clock_reset(c);
+    return sort_unordered(c, kt, records, stride, key_offset, records, n, stride, static_cast<hipStream_t>(stream));
+  }
   if (n > 0 && (!records || !order)) return fail(c, TS_ERR_INVALID_ARG, "null records or order");
   DeviceScope ds(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   clock_reset(c);
   if (n == 0) return TS_OK;
+  if (gather_records && carries(records, n, stride)) {
+    ts_status st = ensure(c, carry_ws(n, kb, stride, true));
+    if (st != TS_OK) return st;
+    return sort_carried(c, kb, mode, records, n, stride, key_offset, order, c->ws, s);
+  }
   const unsigned char* rec = static_cast<const unsigned char*>(records);
   const size_t gbytes = gather_records ? n * stride : 0;
   ts_status st = ensure(c, by_key_ws(n, kb, gbytes));
@@ -845,6 +940,24 @@ ts_status ts_sort_by_key(ts_ctx* c, ts_dtype kt, void* records, size_t n, size_t
     if (e != hipSuccess) return hip_fail(c, e, "hipMemcpyAsync");
   }
   return TS_OK;
+}
+
+ts_status ts_sort_by_key_unordered(ts_ctx* c, ts_dtype kt, const void* keys, size_t key_stride, size_t key_offset,
+                                   void* records, size_t n, size_t record_bytes, void* stream) {
+  if (!c) return TS_ERR_INVALID_ARG;
+  if (!valid_dtype(kt)) return fail(c, TS_ERR_INVALID_ARG, "unknown key dtype");
+  const int kb = ts::dtype_bytes(kt);
+  if (key_stride < size_t(kb) || key_offset + size_t(kb) > key_stride)
+    return fail(c, TS_ERR_INVALID_ARG, "key does not fit inside the key stride");
+  if ((reinterpret_cast<uintptr_t>(keys) + key_offset) % size_t(kb) != 0 || key_stride % size_t(kb) != 0)
+    return fail(c, TS_ERR_INVALID_ARG, "keys must be aligned to the key width");
+  if (record_bytes == 0) return fail(c, TS_ERR_INVALID_ARG, "zero-sized records");
+  if (n > 0 && (!keys || !records)) return fail(c, TS_ERR_INVALID_ARG, "null keys or records");
+  DeviceScope ds(c->device);
+  clock_reset(c);
+  if (n < 2) return TS_OK;
+  return sort_unordered(c, kt, keys, key_stride, key_offset, records, n, record_bytes,
+                        static_cast<hipStream_t>(stream));
 }
 
 ts_status ts_apply_order(ts_ctx* c, const uint32_t* order, size_t order_len, void* other, size_t n,

@@ -70,6 +70,22 @@ struct KernelClock {
 #ifndef TS_SWEEP_SMALL_N
 #define TS_SWEEP_SMALL_N (size_t(1) << 21)
 #endif
+// By-key sorts of 8- and 16-byte records carry the records themselves through
+// the passes (each pass reads the key out of the staged record), with or
+// without the u32 order beside them; LDS holds a tile of records (and order
+// values), no separate keys.
+#ifndef TS_SWEEP_REC8_VALS
+#define TS_SWEEP_REC8_VALS 512, 24
+#endif
+#ifndef TS_SWEEP_REC8
+#define TS_SWEEP_REC8 512, 36
+#endif
+#ifndef TS_SWEEP_REC16_VALS
+#define TS_SWEEP_REC16_VALS 512, 14
+#endif
+#ifndef TS_SWEEP_REC16
+#define TS_SWEEP_REC16 512, 16
+#endif
 // Predecessor tiles one look-back step reads at once.
 #ifndef TS_LOOKBACK_WINDOW
 #define TS_LOOKBACK_WINDOW 4
@@ -78,19 +94,28 @@ template <int THREADS, int IPT> struct ShapeOf {
   static constexpr int threads = THREADS;
   static constexpr int ipt = IPT;
 };
-template <int KB, bool LEARNED = false, bool VALS = false, bool SMALL = false>
+template <int RW, bool VALS>
+using RecShape = std::conditional_t<RW == 4, std::conditional_t<VALS, ShapeOf<TS_SWEEP_REC16_VALS>, ShapeOf<TS_SWEEP_REC16>>,
+                                    std::conditional_t<VALS, ShapeOf<TS_SWEEP_REC8_VALS>, ShapeOf<TS_SWEEP_REC8>>>;
+// RW: 32-bit words of the records a by-key sort carries (0: keys move, the
+// records are gathered afterwards; 2 or 4).
+template <int KB, bool LEARNED = false, bool VALS = false, bool SMALL = false, int RW = 0>
 struct SweepShape
     : std::conditional_t<
-          SMALL, ShapeOf<TS_SWEEP_KEYS_SMALL>,
+          (RW > 0), RecShape<RW, VALS>,
           std::conditional_t<
-              (KB >= 8),
-              std::conditional_t<LEARNED, ShapeOf<TS_SWEEP_LEARNED_WIDE>,
-                                 std::conditional_t<VALS, ShapeOf<TS_SWEEP_VALS_WIDE>, ShapeOf<TS_SWEEP_KEYS_WIDE>>>,
-              std::conditional_t<LEARNED, ShapeOf<TS_SWEEP_LEARNED_NARROW>,
-                                 std::conditional_t<VALS, ShapeOf<TS_SWEEP_VALS_NARROW>,
-                                                    ShapeOf<TS_SWEEP_KEYS_NARROW>>>>> {
+              SMALL, ShapeOf<TS_SWEEP_KEYS_SMALL>,
+              std::conditional_t<
+                  (KB >= 8),
+                  std::conditional_t<LEARNED, ShapeOf<TS_SWEEP_LEARNED_WIDE>,
+                                     std::conditional_t<VALS, ShapeOf<TS_SWEEP_VALS_WIDE>, ShapeOf<TS_SWEEP_KEYS_WIDE>>>,
+                  std::conditional_t<LEARNED, ShapeOf<TS_SWEEP_LEARNED_NARROW>,
+                                     std::conditional_t<VALS, ShapeOf<TS_SWEEP_VALS_NARROW>,
+                                                        ShapeOf<TS_SWEEP_KEYS_NARROW>>>>>> {
   static_assert(SweepShape::threads % 256 == 0, "the first 256 lanes carry one digit each");
-  static_assert(!SMALL || (!LEARNED && !VALS), "small tiles are for keys-only sorts");
+  static_assert(!SMALL || (!LEARNED && !VALS && RW == 0), "small tiles are for keys-only sorts");
+  static_assert(RW == 0 || RW == 2 || RW == 4, "records of 8 or 16 bytes are carried");
+  static_assert(RW == 0 || !LEARNED, "learned digits come from keys");
   static constexpr int tile = SweepShape::threads * SweepShape::ipt;
 };
 
@@ -99,24 +124,35 @@ struct SweepShape
 // sizing and launch shapes always come from the same tile constants.
 //
 // The kind of a sort picks its tile shape: keys only, keys with u32 order
-// values (by-key), or learned bucket digits.
-enum SweepKind : int { kSweepKeys = 0, kSweepVals = 1, kSweepLearned = 2 };
+// values (by-key), learned bucket digits, or 8- / 16-byte records carried
+// with the order (by-key) or without it (by-key sorts beyond u32::MAX).
+enum SweepKind : int {
+  kSweepKeys = 0,
+  kSweepVals = 1,
+  kSweepLearned = 2,
+  kSweepRec8Vals = 3,
+  kSweepRec16Vals = 4,
+  kSweepRec8 = 5,
+  kSweepRec16 = 6,
+};
 // Whether a sort of n keys runs the small-tile shape.
 bool small_sweep(size_t n, bool learned, bool vals);
-// Keys per tile of the shape a sort of n keys of this kind runs.
-size_t sweep_tile_for(size_t n, int key_bytes, bool learned, bool vals);
+// Keys per tile of the shape a sort of n keys of this kind runs (rec_words:
+// the 32-bit words of carried records, 0 when keys move).
+size_t sweep_tile_for(size_t n, int key_bytes, bool learned, bool vals, int rec_words = 0);
 // Upper bound on the tiles of one pass of a sort of n keys: every segment may
 // end in a partial tile.
-uint32_t sweep_tiles(size_t n, int key_bytes, bool learned, bool vals);
+uint32_t sweep_tiles(size_t n, int key_bytes, bool learned, bool vals, int rec_words = 0);
 // Look-back words are 32-bit (2 flag bits, 30 value bits) while every count
 // fits in 30 bits, else 64-bit.
 bool wide_status(size_t n);
 // Bytes of look-back status that one radix sort of n keys of the given kind
 // uses (two alternating rows).
-size_t status_bytes_used(size_t n, int key_bytes, bool learned, bool vals);
+size_t status_bytes_used(size_t n, int key_bytes, bool learned, bool vals, int rec_words = 0);
 // Bytes of look-back status to reserve so that every radix sort of AT MOST n
-// keys of this width, of any kind, fits: shorter sorts may run the small
-// tiles, which need more rows per key than the large ones.
+// keys of this width, of the keys, values or learned kinds, fits: shorter
+// sorts may run the small tiles, which need more rows per key than the large
+// ones.  (Record-carrying sorts reserve status_bytes_used of their own kind.)
 size_t status_bytes(size_t n, int key_bytes);
 
 // One LSD sort on the device.  Keys start in keys_a (plain sort) or are read
@@ -132,6 +168,13 @@ struct RadixJob {
   uint32_t* vals_a = nullptr;  // by-key: order ping-pong
   uint32_t* vals_b = nullptr;
   uint32_t* final_vals = nullptr;  // by-key: the last pass writes order here
+  // By-key sorts of 8- or 16-byte records (stride 4 * rec_words, aligned to
+  // it): the records move through the passes, ping-ponging between
+  // `records` (where they end, in stable key order) and recs_b, and every
+  // pass reads the key at key_offset of each record; keys_a / keys_b are
+  // unused.  With final_vals the order is written too.
+  int rec_words = 0;
+  void* recs_b = nullptr;
   SortMeta* meta = nullptr;
   void* status = nullptr;
   size_t status_cap = 0;  // bytes at `status`; a sort needing more is refused
@@ -172,6 +215,14 @@ hipError_t small_sort(int key_bytes, int mode, void* keys, size_t n, const unsig
 // out[i] = src[order[i]] for n elements of elem_bytes bytes.
 hipError_t gather(const uint32_t* order, const void* src, void* dst, size_t n, size_t elem_bytes, hipStream_t s,
                   KernelClock* clk);
+// The same with the positions taken from words 2-3 of n 16-byte records
+// (pack_key_pos records, sorted by key).
+hipError_t gather_packed(const void* packed, const void* src, void* dst, size_t n, size_t elem_bytes, hipStream_t s,
+                         KernelClock* clk);
+// out[i] = (the key_bytes-wide key at keys + i * stride + key_offset,
+// zero-extended to 64 bits; i as 64 bits): 16 bytes per element.
+hipError_t pack_key_pos(int key_bytes, const void* keys, size_t stride, size_t key_offset, void* out, size_t n,
+                        hipStream_t s);
 
 // Batched segmented sort (segmented.hip).  Segments of up to
 // seg_lds_max_len keys are sorted inside one workgroup each; longer ones are

@@ -67,7 +67,8 @@ __global__ void k_init(SortMeta* m, int P, int gate) {
 // segment into LDS and adds its counts to the global table once.  The
 // forward key transform is fused into the load; the by-key path (REC) reads
 // keys out of records and writes the transformed key array the scatter
-// passes start from.
+// passes start from (unless the passes carry the records themselves:
+// keys_out null).
 //
 // A digit (with its segment bits) on which all of a wave's keys of one step
 // agree -- constant digits, sorted runs -- is counted with one LDS add
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
       }
     }
     to_keys(k, mode);
-    if (REC) {
+    if (REC && keys_out) {
 #pragma unroll
       for (int r = 0; r < UN; ++r) {
         const size_t i = base + size_t(r) * kHistThreads + tid;
@@ -506,17 +507,59 @@ template <class U> struct SweepArgs {
   const LearnedModel* model;  // learned partition: digits of the key's bucket
   int atomic_rank;            // rank with LDS atomics (rank_in_wave_atomic), else ballots
   int gate;                   // the second sort after a divert: run only when meta->fallback is set
+  // Carried records (RW > 0): ping-pong halves (recs_a holds the input and
+  // the result), and where each record's key lies: 32-bit word key_word,
+  // shifted right by key_shift bits; rec_mode is the key's transform.
+  void* recs_a;
+  void* recs_b;
+  uint32_t key_word, key_shift;
+  int rec_mode;
 };
+
+// A carried record: RW 32-bit words.
+template <int RW> struct RecOf {
+  using T = uint32_t;
+};
+template <> struct RecOf<2> {
+  using T = uint2;
+};
+template <> struct RecOf<4> {
+  using T = uint4;
+};
+
+// The key stored in a carried record at 32-bit word `word`, bit `shift`
+// (keys lie aligned to their width, so an 8-byte key starts at word 0 or 2).
+// The selects are uniform: the word index is the same for every record.
+template <class U, int RW>
+__device__ __forceinline__ U rec_key(const typename RecOf<RW>::T& r, uint32_t word, uint32_t shift) {
+  if constexpr (RW == 2) {
+    if constexpr (sizeof(U) == 8)
+      return U((uint64_t(r.y) << 32) | r.x);
+    else
+      return U((word ? r.y : r.x) >> shift);
+  } else if constexpr (RW == 4) {
+    if constexpr (sizeof(U) == 8) {
+      const uint32_t lo = word ? r.z : r.x, hi = word ? r.w : r.y;
+      return U((uint64_t(hi) << 32) | lo);
+    } else {
+      const uint32_t w = word == 0 ? r.x : word == 1 ? r.y : word == 2 ? r.z : r.w;
+      return U(w >> shift);
+    }
+  } else {
+    return U(0);
+  }
+}
 
 // About two seconds of s_sleep(1) polling.
 constexpr uint32_t kSpinLimit = 1u << 26;
 
 // LDS of one scatter tile, declared once in the kernel.
-template <class U, bool VALS, int IPT, bool LEARNED = false, bool SMALL = false> struct SweepLds {
-  static constexpr int NW = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads / 64;
-  static constexpr int TILE = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads * IPT;
+template <class U, bool VALS, int IPT, bool LEARNED = false, bool SMALL = false, int RW = 0> struct SweepLds {
+  static constexpr int NW = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads / 64;
+  static constexpr int TILE = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads * IPT;
   uint32_t whist[NW][256];  // per-wave digit counts, then staging bases
-  U keys[TILE];
+  U keys[RW ? 1 : TILE];
+  typename RecOf<RW>::T recs[RW ? TILE : 1];
   uint32_t vals[VALS ? TILE : 1];
   unsigned long long knots[LEARNED ? kKnots : 1];
   unsigned long long gofs[256];
@@ -528,14 +571,14 @@ template <class U, bool VALS, int IPT, bool LEARNED = false, bool SMALL = false>
 // Everything a tile does after it knows where it is: load, publish its digit
 // counts, rank, look back, stage, store.  FULL tiles (all but each segment's
 // last) run without any per-key bounds test.
-template <class U, bool VALS, class S, bool FULL, bool LEARNED, bool SMALL>
-__device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS, SMALL>::ipt, LEARNED, SMALL>& L,
-                                           const SweepArgs<U>& a,
-                                           S* __restrict__ status, int pass, unsigned shift, bool first, bool last,
-                                           size_t tile_base, uint32_t valid, size_t first_slot, size_t slot,
-                                           bool first_round, unsigned long long digit_base) {
-  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::ipt;
-  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads;
+template <class U, bool VALS, class S, bool FULL, bool LEARNED, bool SMALL, int RW>
+__device__ __forceinline__ void run_tile(
+    SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt, LEARNED, SMALL, RW>& L,
+    const SweepArgs<U>& a, S* __restrict__ status, int pass, unsigned shift, bool first, bool last, size_t tile_base,
+    uint32_t valid, size_t first_slot, size_t slot, bool first_round, unsigned long long digit_base) {
+  using R = typename RecOf<RW>::T;
+  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
   constexpr int VB = Flags<S>::kBits;
@@ -552,12 +595,24 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
 
   U k[IPT];
   uint32_t v[VALS ? IPT : 1];
+  R r[RW ? IPT : 1];
   {
-    const U* __restrict__ ts = src + tile_base;
+    if constexpr (RW > 0) {
+      const R* __restrict__ rs = static_cast<const R*>((pass & 1) ? a.recs_b : a.recs_a) + tile_base;
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t li = lb + uint32_t(j) * 64;
-      k[j] = ts[FULL || li < valid ? li : valid - 1];
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t li = lb + uint32_t(j) * 64;
+        r[j] = rs[FULL || li < valid ? li : valid - 1];
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) k[j] = to_key(rec_key<U, RW>(r[j], a.key_word, a.key_shift), a.rec_mode);
+    } else {
+      const U* __restrict__ ts = src + tile_base;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t li = lb + uint32_t(j) * 64;
+        k[j] = ts[FULL || li < valid ? li : valid - 1];
+      }
     }
     if (VALS) {
       if (first) {
@@ -572,7 +627,7 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
         }
       }
     }
-    if (first) to_keys(k, a.mode_in);
+    if (RW == 0 && first) to_keys(k, a.mode_in);
   }
   uint32_t bk[LEARNED ? IPT : 1];
   if constexpr (LEARNED) {
@@ -641,7 +696,10 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
     if (in_tile(j)) {
       const uint32_t d = dig(j);
       const uint32_t pos = L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
-      L.keys[pos] = k[j];
+      if constexpr (RW > 0)
+        L.recs[pos] = r[j];
+      else
+        L.keys[pos] = k[j];
       if (VALS) L.vals[pos] = v[j];
     }
     // Keep the scheduler from hoisting every iteration's LDS read ahead of
@@ -715,6 +773,24 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
       if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
   };
+  if constexpr (RW > 0) {
+    // Records leave as they came; the key is read out of each again.
+    R* __restrict__ rdst = static_cast<R*>((pass & 1) ? a.recs_a : a.recs_b);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t s = uint32_t(j) * THREADS + tid;
+      if (FULL || s < valid) {
+        const R rr = L.recs[s];
+        const U kk = to_key(rec_key<U, RW>(rr, a.key_word, a.key_shift), a.rec_mode);
+        const size_t o = size_t(L.gofs[digit_of(kk, shift)]) + s;
+        rdst[o] = rr;
+        if (VALS) vdst[o] = L.vals[s];
+      }
+      if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    (void)TILE;
+    return;
+  }
   const bool write_keys = !(VALS && last && a.final_vals);
   const int out_mode = last ? a.mode_out : int(kIdentity);
   if (!write_keys)
@@ -745,17 +821,19 @@ __device__ __forceinline__ void run_tile(SweepLds<U, VALS, SweepShape<sizeof(U),
 //
 // Pass 0 of a by-key sort generates the values (positions 0..n-1); the last
 // pass applies the inverse key transform, and in a by-key sort writes only the
-// order to final_vals.  PERSIST: a grid smaller than the tile count whose
+// order to final_vals.  RW > 0: the by-key sort's 8- or 16-byte records are
+// staged and written instead of keys, each pass reading the key out of them,
+// so that the records end in order without a gather.  PERSIST: a grid smaller than the tile count whose
 // workgroups claim tiles until none is left (the gated second sort, which
 // then costs a few microseconds when it does not run).
-template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false, bool PERSIST = false>
-__global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads)) void k_sweep(
+template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false, bool PERSIST = false, int RW = 0>
+__global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads)) void k_sweep(
     SweepArgs<U> a, S* __restrict__ status, int pass) {
-  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::ipt;
-  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threads;
+  constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt;
+  constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads;
   constexpr int NW = THREADS / 64;
   constexpr int TILE = THREADS * IPT;
-  __shared__ SweepLds<U, VALS, IPT, LEARNED, SMALL> L;
+  __shared__ SweepLds<U, VALS, IPT, LEARNED, SMALL, RW> L;
 
   const SortMeta* __restrict__ meta = a.meta;
   if (a.gate && !meta->fallback) return;
@@ -819,11 +897,11 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL>::threa
     // in this segment.
     const unsigned long long digit_base = tid < 256 ? meta->pbase[pass][seg][tid] : 0ull;
     if (valid == uint32_t(TILE))
-      run_tile<U, VALS, S, true, LEARNED, SMALL>(L, a, status, pass, shift, first, last, tile_base, valid, first_slot,
-                                                   slot, round == 0, digit_base);
+      run_tile<U, VALS, S, true, LEARNED, SMALL, RW>(L, a, status, pass, shift, first, last, tile_base, valid,
+                                                       first_slot, slot, round == 0, digit_base);
     else
-      run_tile<U, VALS, S, false, LEARNED, SMALL>(L, a, status, pass, shift, first, last, tile_base, valid,
-                                                    first_slot, slot, round == 0, digit_base);
+      run_tile<U, VALS, S, false, LEARNED, SMALL, RW>(L, a, status, pass, shift, first, last, tile_base, valid,
+                                                        first_slot, slot, round == 0, digit_base);
     if (!PERSIST) return;
     __syncthreads();  // the tile's LDS is reused by the next claim
   }
@@ -862,6 +940,15 @@ template <class U>
 __global__ void k_copy_back(const SortMeta* m, const U* __restrict__ src, U* __restrict__ dst, size_t n) {
   if (m->reverse) return reverse_keys(dst, n);
   if (m->skip || (m->n_live & 1u) == 0) return;
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = g; i < n; i += gs) dst[i] = src[i];
+}
+
+// A record-carrying sort with an odd number of live passes ends in the
+// scratch half.
+template <class R>
+__global__ void k_copy_back_records(const SortMeta* m, const R* __restrict__ src, R* __restrict__ dst, size_t n) {
+  if ((m->n_live & 1u) == 0) return;
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
   for (size_t i = g; i < n; i += gs) dst[i] = src[i];
 }
@@ -1108,8 +1195,8 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(SortMeta* m, U* __res
 }
 
 // Length of the first pass's position segments (a whole number of tiles).
-size_t first_segment(size_t n, int key_bytes, bool learned, bool vals) {
-  const size_t tile = sweep_tile_for(n, key_bytes, learned, vals);
+size_t first_segment(size_t n, int key_bytes, bool learned, bool vals, int rec_words) {
+  const size_t tile = sweep_tile_for(n, key_bytes, learned, vals, rec_words);
   const size_t per = (n + kSegments - 1) / kSegments;
   return ((per + tile - 1) / tile) * tile;
 }
@@ -1124,10 +1211,18 @@ size_t first_segment(size_t n, int key_bytes, bool learned, bool vals) {
 
 // gate: the second sort after a divert (persistent launches for keys-only
 // 8-byte keys, run only when the finish found an out-of-order long run).
+// One pass of a record-carrying sort (RW words per record).
+template <class U, class S, bool VALS, int RW>
+void launch_rec_sweep(uint32_t tiles, hipStream_t s, const SweepArgs<U>& a, void* status, int p) {
+  hipLaunchKernelGGL((k_sweep<U, VALS, S, false, false, false, RW>), dim3(tiles),
+                     dim3(SweepShape<sizeof(U), false, VALS, false, RW>::threads), 0, s, a, static_cast<S*>(status), p);
+}
+
 template <class U, class S, bool LEARNED>
 hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool gate, bool divert = false) {
   const bool vals = j.final_vals != nullptr;
-  const uint32_t tiles = sweep_tiles(j.n, sizeof(U), LEARNED, vals);
+  const int rw = LEARNED ? 0 : j.rec_words;
+  const uint32_t tiles = sweep_tiles(j.n, sizeof(U), LEARNED, vals, rw);
   SweepArgs<U> a;
   a.keys_a = static_cast<U*>(j.keys_a);
   a.keys_b = static_cast<U*>(j.keys_b);
@@ -1145,10 +1240,24 @@ hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool g
   a.model = j.model;
   a.atomic_rank = j.atomic_rank ? 1 : 0;
   a.gate = gate ? 1 : 0;
-  const double bytes = double(j.n) * (2.0 * sizeof(U) + (vals ? 8.0 : 0.0));
+  a.recs_a = const_cast<unsigned char*>(j.records);
+  a.recs_b = j.recs_b;
+  a.key_word = uint32_t(j.key_offset / 4);
+  a.key_shift = uint32_t(8 * (j.key_offset % 4));
+  a.rec_mode = j.mode;
+  const double bytes = double(j.n) * (2.0 * (rw ? 4.0 * rw : double(sizeof(U))) + (vals ? 8.0 : 0.0));
   for (int p = 0; p < DigitSource<U, LEARNED>::P; ++p) {
-    if (clk) clk->begin(s, gate ? "k_sweep_fallback" : "k_sweep", bytes);
-    if constexpr (LEARNED) {
+    if (clk) clk->begin(s, gate ? "k_sweep_fallback" : rw ? "k_sweep_records" : "k_sweep", bytes);
+    if (rw) {
+      if (rw == 2 && vals)
+        launch_rec_sweep<U, S, true, 2>(tiles, s, a, j.status, p);
+      else if (rw == 2)
+        launch_rec_sweep<U, S, false, 2>(tiles, s, a, j.status, p);
+      else if (vals)
+        launch_rec_sweep<U, S, true, 4>(tiles, s, a, j.status, p);
+      else
+        launch_rec_sweep<U, S, false, 4>(tiles, s, a, j.status, p);
+    } else if constexpr (LEARNED) {
       hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U), true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
     } else if (vals) {
@@ -1178,14 +1287,20 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   constexpr int P = DigitSource<U, LEARNED>::P;
   constexpr int KB = int(sizeof(U));
   const bool vals = j.final_vals != nullptr;
+  const int rw = j.rec_words;
+  if (rw && (LEARNED || !j.records || !j.recs_b || j.stride != size_t(4 * rw) ||
+             reinterpret_cast<uintptr_t>(j.records) % j.stride != 0 ||
+             reinterpret_cast<uintptr_t>(j.recs_b) % j.stride != 0 || j.key_offset + KB > j.stride ||
+             j.key_offset % KB != 0))
+    return hipErrorInvalidValue;
   // The look-back rows this sort's launches index must lie inside the
   // caller's status area.
-  if (!j.status || status_bytes_used(j.n, KB, LEARNED, vals) > j.status_cap) return hipErrorInvalidValue;
+  if (!j.status || status_bytes_used(j.n, KB, LEARNED, vals, rw) > j.status_cap) return hipErrorInvalidValue;
   const bool divert = !LEARNED && !vals && !j.records && KB == 8 && j.divert && !small_sweep(j.n, false, false);
   hipLaunchKernelGGL(k_init, dim3(16), dim3(256), 0, s, j.meta, P, 0);
   // Histogram: blocks_per_seg workgroups per position segment, each over a
   // contiguous chunk of it.
-  const size_t seg0 = first_segment(j.n, KB, LEARNED, vals);
+  const size_t seg0 = first_segment(j.n, KB, LEARNED, vals, rw);
   // Histogram workgroups per CU.  The learned histogram holds three digit
   // tables (and the knots), not eight, so more fit a CU;
   // TS_HIST_BLOCKS_PER_CU_LEARNED counts 256-lane workgroups.
@@ -1196,10 +1311,10 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
   size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
   per_block = std::max(step, ((per_block + step - 1) / step) * step);
-  const uint32_t tiles = sweep_tiles(j.n, KB, LEARNED, vals);
+  const uint32_t tiles = sweep_tiles(j.n, KB, LEARNED, vals, rw);
   const size_t zero_words = size_t(tiles) * 256 * (wide_status(j.n) ? 2 : 1);
   const dim3 hgrid(blocks_per_seg * kSegments);
-  const uint32_t tile = uint32_t(sweep_tile_for(j.n, KB, LEARNED, vals));
+  const uint32_t tile = uint32_t(sweep_tile_for(j.n, KB, LEARNED, vals, rw));
   // The order check does not apply to bucket digits, and only keys without
   // values may be reversed (a stable order of non-increasing keys with ties
   // is no reversal).
@@ -1223,7 +1338,8 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
                          static_cast<uint32_t*>(j.status), zero_words, j.model, co, gate, j.fingerprint);
     else if (j.records)
       hipLaunchKernelGGL((k_hist<U, true>), hgrid, dim3(kHistThreads), 0, s, nullptr, j.records, j.stride,
-                         j.key_offset, static_cast<U*>(j.keys_a), j.n, j.mode, seg0, blocks_per_seg, per_block, j.meta,
+                         j.key_offset, rw ? nullptr : static_cast<U*>(j.keys_a), j.n, j.mode, seg0, blocks_per_seg,
+                         per_block, j.meta,
                          static_cast<uint32_t*>(j.status), zero_words, nullptr, co, gate, nullptr);
     else
       hipLaunchKernelGGL((k_hist<U, false>), hgrid, dim3(kHistThreads), 0, s, static_cast<const U*>(j.keys_a),
@@ -1260,9 +1376,17 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
     }
   }
   const unsigned cb = unsigned(std::min<size_t>((j.n + 255) / 256, size_t(j.num_cus) * 8));
+  if (rw == 2)
+    hipLaunchKernelGGL(k_copy_back_records<uint2>, dim3(cb ? cb : 1), dim3(256), 0, s, j.meta,
+                       static_cast<const uint2*>(j.recs_b), reinterpret_cast<uint2*>(const_cast<unsigned char*>(j.records)),
+                       j.n);
+  else if (rw == 4)
+    hipLaunchKernelGGL(k_copy_back_records<uint4>, dim3(cb ? cb : 1), dim3(256), 0, s, j.meta,
+                       static_cast<const uint4*>(j.recs_b), reinterpret_cast<uint4*>(const_cast<unsigned char*>(j.records)),
+                       j.n);
   if (j.final_vals) {
     hipLaunchKernelGGL(k_iota_if_dead, dim3(cb ? cb : 1), dim3(256), 0, s, j.meta, j.final_vals, j.n);
-  } else {
+  } else if (!rw) {
     hipLaunchKernelGGL((k_copy_back<U>), dim3(cb ? cb : 1), dim3(256), 0, s, j.meta, static_cast<const U*>(j.keys_b),
                        static_cast<U*>(j.keys_a), j.n);
   }
@@ -1318,7 +1442,11 @@ __global__ __launch_bounds__(512) void k_rank_check(uint32_t* bad) {
 
 // ---------------------------------------------------------- tile geometry
 namespace {
-size_t tile_of(int key_bytes, bool learned, bool vals, bool small) {
+size_t tile_of(int key_bytes, bool learned, bool vals, bool small, int rec_words) {
+  if (rec_words == 2)
+    return size_t(vals ? SweepShape<4, false, true, false, 2>::tile : SweepShape<4, false, false, false, 2>::tile);
+  if (rec_words == 4)
+    return size_t(vals ? SweepShape<4, false, true, false, 4>::tile : SweepShape<4, false, false, false, 4>::tile);
   if (small) return size_t(key_bytes < 8 ? SweepShape<4, false, false, true>::tile : SweepShape<8, false, false, true>::tile);
   if (key_bytes < 8)
     return size_t(learned ? SweepShape<4, true>::tile : vals ? SweepShape<4, false, true>::tile : SweepShape<4>::tile);
@@ -1329,19 +1457,19 @@ size_t status_row_bytes(size_t n) { return size_t(256) * (wide_status(n) ? 8 : 4
 
 bool small_sweep(size_t n, bool learned, bool vals) { return !learned && !vals && n <= TS_SWEEP_SMALL_N; }
 
-size_t sweep_tile_for(size_t n, int key_bytes, bool learned, bool vals) {
-  return tile_of(key_bytes, learned, vals, small_sweep(n, learned, vals));
+size_t sweep_tile_for(size_t n, int key_bytes, bool learned, bool vals, int rec_words) {
+  return tile_of(key_bytes, learned, vals, rec_words == 0 && small_sweep(n, learned, vals), rec_words);
 }
 
-uint32_t sweep_tiles(size_t n, int key_bytes, bool learned, bool vals) {
-  const size_t tile = sweep_tile_for(n, key_bytes, learned, vals);
+uint32_t sweep_tiles(size_t n, int key_bytes, bool learned, bool vals, int rec_words) {
+  const size_t tile = sweep_tile_for(n, key_bytes, learned, vals, rec_words);
   return uint32_t((n + tile - 1) / tile + kSegments);
 }
 
 bool wide_status(size_t n) { return n >= (size_t(1) << 30); }
 
-size_t status_bytes_used(size_t n, int key_bytes, bool learned, bool vals) {
-  return size_t(2) * sweep_tiles(n, key_bytes, learned, vals) * status_row_bytes(n);
+size_t status_bytes_used(size_t n, int key_bytes, bool learned, bool vals, int rec_words) {
+  return size_t(2) * sweep_tiles(n, key_bytes, learned, vals, rec_words) * status_row_bytes(n);
 }
 
 size_t status_bytes(size_t n, int key_bytes) {

@@ -17,32 +17,88 @@ inline unsigned grid_for(size_t n, size_t per_thread = 1) {
 }
 
 // by_key.rs:315-334: out[i] = src[order[i]].  Coalesced order read and
-// output write; the source read is random.
-template <class E>
-__global__ void k_gather(const uint32_t* __restrict__ order, const E* __restrict__ src, E* __restrict__ dst,
-                         size_t n) {
-  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
-  for (size_t i = g; i < n; i += gs) dst[i] = src[order[i]];
+// output write; the source read is random.  The order is u32 positions, or
+// (Packed) the 16-byte (key, u64 position) records of a sort beyond
+// u32::MAX, whose words 2 and 3 hold the position.
+struct Packed {
+  uint4 v;
+};
+__device__ __forceinline__ size_t order_at(const uint32_t* __restrict__ o, size_t i) { return o[i]; }
+__device__ __forceinline__ size_t order_at(const Packed* __restrict__ o, size_t i) {
+  const uint4 p = o[i].v;
+  return size_t(p.z) | (size_t(p.w) << 32);
 }
 
-__global__ void k_gather_bytes(const uint32_t* __restrict__ order, const unsigned char* __restrict__ src,
+template <class E, class O>
+__global__ void k_gather(const O* __restrict__ order, const E* __restrict__ src, E* __restrict__ dst, size_t n) {
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = g; i < n; i += gs) dst[i] = src[order_at(order, i)];
+}
+
+template <class O>
+__global__ void k_gather_bytes(const O* __restrict__ order, const unsigned char* __restrict__ src,
                                unsigned char* __restrict__ dst, size_t n, size_t eb) {
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
   for (size_t i = g; i < n; i += gs) {
-    const unsigned char* s = src + size_t(order[i]) * eb;
+    const unsigned char* s = src + order_at(order, i) * eb;
     unsigned char* d = dst + i * eb;
     for (size_t b = 0; b < eb; ++b) d[b] = s[b];
   }
 }
 
-__global__ void k_gather_words(const uint32_t* __restrict__ order, const uint32_t* __restrict__ src,
+template <class O>
+__global__ void k_gather_words(const O* __restrict__ order, const uint32_t* __restrict__ src,
                                uint32_t* __restrict__ dst, size_t n, size_t words) {
   const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
   for (size_t i = g; i < n; i += gs) {
-    const uint32_t* s = src + size_t(order[i]) * words;
+    const uint32_t* s = src + order_at(order, i) * words;
     uint32_t* d = dst + i * words;
     for (size_t b = 0; b < words; ++b) d[b] = s[b];
   }
+}
+
+// The (raw key bits, position) record of each element for a by-key sort
+// beyond u32::MAX: words 0-1 the key (zero-extended), words 2-3 its index.
+template <class U>
+__global__ void k_pack_key_pos(const unsigned char* __restrict__ keys, size_t stride, size_t key_off,
+                               uint4* __restrict__ out, size_t n) {
+  const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x, gs = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = g; i < n; i += gs) {
+    const uint64_t k = uint64_t(*reinterpret_cast<const U*>(keys + i * stride + key_off));
+    out[i] = make_uint4(uint32_t(k), uint32_t(k >> 32), uint32_t(i), uint32_t(uint64_t(i) >> 32));
+  }
+}
+
+template <class O>
+hipError_t gather_by(const O* order, const void* src, void* dst, size_t n, size_t eb, hipStream_t s,
+                     KernelClock* clk) {
+  if (n == 0) return hipSuccess;
+  const uintptr_t al = reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst);
+  const unsigned grid = grid_for(n);
+  if (clk) clk->begin(s, "k_gather", double(n) * (double(sizeof(O)) + 2.0 * eb));
+  if (eb == 16 && al % 16 == 0)
+    hipLaunchKernelGGL((k_gather<uint4, O>), dim3(grid), dim3(kThreads), 0, s, order, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n);
+  else if (eb == 8 && al % 8 == 0)
+    hipLaunchKernelGGL((k_gather<uint64_t, O>), dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const uint64_t*>(src), static_cast<uint64_t*>(dst), n);
+  else if (eb == 4 && al % 4 == 0)
+    hipLaunchKernelGGL((k_gather<uint32_t, O>), dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), n);
+  else if (eb == 2 && al % 2 == 0)
+    hipLaunchKernelGGL((k_gather<uint16_t, O>), dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), n);
+  else if (eb == 1)
+    hipLaunchKernelGGL((k_gather<uint8_t, O>), dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const uint8_t*>(src), static_cast<uint8_t*>(dst), n);
+  else if (eb % 4 == 0 && al % 4 == 0)
+    hipLaunchKernelGGL(k_gather_words<O>, dim3(grid), dim3(kThreads), 0, s, order, static_cast<const uint32_t*>(src),
+                       static_cast<uint32_t*>(dst), n, eb / 4);
+  else
+    hipLaunchKernelGGL(k_gather_bytes<O>, dim3(grid), dim3(kThreads), 0, s, order,
+                       static_cast<const unsigned char*>(src), static_cast<unsigned char*>(dst), n, eb);
+  if (clk) clk->end(s);
+  return hipGetLastError();
 }
 
 __global__ void k_splitmix(void* out, size_t n, uint64_t seed, int kind) {
@@ -130,32 +186,27 @@ __global__ __launch_bounds__(kThreads) void k_check(const U* __restrict__ x, siz
 
 hipError_t gather(const uint32_t* order, const void* src, void* dst, size_t n, size_t eb, hipStream_t s,
                   KernelClock* clk) {
+  return gather_by(order, src, dst, n, eb, s, clk);
+}
+
+hipError_t gather_packed(const void* packed, const void* src, void* dst, size_t n, size_t eb, hipStream_t s,
+                         KernelClock* clk) {
+  return gather_by(static_cast<const Packed*>(packed), src, dst, n, eb, s, clk);
+}
+
+hipError_t pack_key_pos(int key_bytes, const void* keys, size_t stride, size_t key_offset, void* out, size_t n,
+                        hipStream_t s) {
   if (n == 0) return hipSuccess;
-  const uintptr_t al = reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst);
   const unsigned grid = grid_for(n);
-  if (clk) clk->begin(s, "k_gather", double(n) * (4.0 + 2.0 * eb));
-  if (eb == 16 && al % 16 == 0)
-    hipLaunchKernelGGL(k_gather<uint4>, dim3(grid), dim3(kThreads), 0, s, order, static_cast<const uint4*>(src),
-                       static_cast<uint4*>(dst), n);
-  else if (eb == 8 && al % 8 == 0)
-    hipLaunchKernelGGL(k_gather<uint64_t>, dim3(grid), dim3(kThreads), 0, s, order,
-                       static_cast<const uint64_t*>(src), static_cast<uint64_t*>(dst), n);
-  else if (eb == 4 && al % 4 == 0)
-    hipLaunchKernelGGL(k_gather<uint32_t>, dim3(grid), dim3(kThreads), 0, s, order,
-                       static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), n);
-  else if (eb == 2 && al % 2 == 0)
-    hipLaunchKernelGGL(k_gather<uint16_t>, dim3(grid), dim3(kThreads), 0, s, order,
-                       static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), n);
-  else if (eb == 1)
-    hipLaunchKernelGGL(k_gather<uint8_t>, dim3(grid), dim3(kThreads), 0, s, order,
-                       static_cast<const uint8_t*>(src), static_cast<uint8_t*>(dst), n);
-  else if (eb % 4 == 0 && al % 4 == 0)
-    hipLaunchKernelGGL(k_gather_words, dim3(grid), dim3(kThreads), 0, s, order, static_cast<const uint32_t*>(src),
-                       static_cast<uint32_t*>(dst), n, eb / 4);
-  else
-    hipLaunchKernelGGL(k_gather_bytes, dim3(grid), dim3(kThreads), 0, s, order,
-                       static_cast<const unsigned char*>(src), static_cast<unsigned char*>(dst), n, eb);
-  if (clk) clk->end(s);
+  const unsigned char* k = static_cast<const unsigned char*>(keys);
+  uint4* o = static_cast<uint4*>(out);
+  switch (key_bytes) {
+    case 1: hipLaunchKernelGGL(k_pack_key_pos<uint8_t>, dim3(grid), dim3(kThreads), 0, s, k, stride, key_offset, o, n); break;
+    case 2: hipLaunchKernelGGL(k_pack_key_pos<uint16_t>, dim3(grid), dim3(kThreads), 0, s, k, stride, key_offset, o, n); break;
+    case 4: hipLaunchKernelGGL(k_pack_key_pos<uint32_t>, dim3(grid), dim3(kThreads), 0, s, k, stride, key_offset, o, n); break;
+    case 8: hipLaunchKernelGGL(k_pack_key_pos<uint64_t>, dim3(grid), dim3(kThreads), 0, s, k, stride, key_offset, o, n); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
