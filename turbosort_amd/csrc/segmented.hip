@@ -231,20 +231,21 @@ template <class U, int T, int IPT, bool VALS> __global__ __launch_bounds__(T) vo
     // real key, stay last.
     {
       U kin[IPT];
+      uint32_t vin[VALS ? IPT : 1];
 #pragma unroll
       for (int j = 0; j < IPT; ++j) kin[j] = keys[min(uint32_t(j) * T + tid, len - 1)];
       if (VALS) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-          const uint32_t i = uint32_t(j) * T + tid;
-          L.vals[sb + j * TP] = vals[min(i, len - 1)];
-        }
+        for (int j = 0; j < IPT; ++j) vin[j] = vals[min(uint32_t(j) * T + tid, len - 1)];
       }
+      // Keys and values in flight together: one memory round trip.
+      __builtin_amdgcn_sched_barrier(0);
       to_keys(kin, a.mode);
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
         const uint32_t i = uint32_t(j) * T + tid;
         L.keys[sb + j * TP] = i < len ? kin[j] : U(~U(0));
+        if (VALS) L.vals[sb + j * TP] = vin[j];
       }
     }
     if (tid == 0) {
@@ -447,17 +448,21 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
     // padding keys are the largest key and, after every real key, stay last.
     {
       U kin[IPT];
+      uint32_t vin[VALS ? IPT : 1];
 #pragma unroll
       for (int j = 0; j < IPT; ++j) kin[j] = keys[min(uint32_t(j) * T + tid, len - 1)];
       if (VALS) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) L.vals[uint32_t(j) * T + tid] = vals[min(uint32_t(j) * T + tid, len - 1)];
+        for (int j = 0; j < IPT; ++j) vin[j] = vals[min(uint32_t(j) * T + tid, len - 1)];
       }
+      // Keys and values in flight together: one memory round trip.
+      __builtin_amdgcn_sched_barrier(0);
       to_keys(kin, a.mode);
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
         const uint32_t i = uint32_t(j) * T + tid;
         L.keys[i] = i < len ? kin[j] : U(~U(0));
+        if (VALS) L.vals[i] = vin[j];
       }
     }
     if (tid == 0) {
