@@ -184,6 +184,26 @@ struct LearnedModel {
   unsigned long long xhi;   // coordinate of the largest sample key (below 2^32)
 };
 
+// The coordinate parameters of a LearnedModel, copied into registers once by
+// the kernels that evaluate the model for every key (read through the
+// model's pointer, they would be reloaded -- and waited for -- per key).
+struct LearnedCoord {
+  unsigned long long kmin, xhi;
+  uint32_t linear, shift, cmul, cshift, bucket_shift;
+};
+
+__host__ __device__ __forceinline__ LearnedCoord coord_of(const LearnedModel& m) {
+  LearnedCoord c;
+  c.kmin = m.kmin;
+  c.xhi = m.xhi;
+  c.linear = m.linear;
+  c.shift = m.shift;
+  c.cmul = m.cmul;
+  c.cshift = m.cshift;
+  c.bucket_shift = m.bucket_shift;
+  return c;
+}
+
 #if defined(__HIP_DEVICE_COMPILE__)
 #define TS_CLZ64(x) __clzll((long long)(x))
 #else
@@ -198,7 +218,8 @@ __host__ __device__ __forceinline__ uint32_t log2_fixed(unsigned long long k) {
   return uint32_t(63 - lz) << 23 | (uint32_t((k << lz) >> 40) & 0x7FFFFFu);
 }
 
-__host__ __device__ __forceinline__ unsigned long long learned_x(unsigned long long key, const LearnedModel& m) {
+template <class M>
+__host__ __device__ __forceinline__ unsigned long long learned_x(unsigned long long key, const M& m) {
   const unsigned long long d = key > m.kmin ? key - m.kmin : 0ull;
   return m.linear ? d >> m.shift : log2_fixed(d);
 }
@@ -218,7 +239,8 @@ __host__ __device__ __forceinline__ void learned_scale(unsigned long long xhi, u
 // The model coordinate of a radix key: u in [0, kKnots << kKnotFracBits).
 // x * cmul >> cshift, computed as the high word of (x << (32 - cshift)) *
 // cmul: the same value (x << (32 - cshift) stays below 2^32), one multiply.
-__host__ __device__ __forceinline__ uint32_t learned_coord(unsigned long long key, const LearnedModel& m) {
+template <class M>
+__host__ __device__ __forceinline__ uint32_t learned_coord(unsigned long long key, const M& m) {
   unsigned long long x = learned_x(key, m);
   x = x > m.xhi ? m.xhi : x;
   const uint32_t xs = uint32_t(x) << (32 - m.cshift);

@@ -396,7 +396,7 @@ __global__ void k_learn_bounds(const U* __restrict__ data, size_t n, int mode, c
   __shared__ unsigned long long knots[kKnots];
   for (unsigned i = threadIdx.x; i < unsigned(kKnots); i += blockDim.x) knots[i] = knot_pair(model->knots, int(i));
   __syncthreads();
-  const LearnedModel& m = *model;
+  const LearnedCoord m = coord_of(*model);
   auto bucket = [&](U x) {
     return learned_cdf_pairs(learned_coord((unsigned long long)to_key<U>(x, mode), m), knots) >> m.bucket_shift;
   };
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(256) void k_learn_bounds_search(const U* __restrict
   __shared__ unsigned long long knots[kKnots];
   for (unsigned i = threadIdx.x; i < unsigned(kKnots); i += blockDim.x) knots[i] = knot_pair(model->knots, int(i));
   __syncthreads();
-  const LearnedModel& m = *model;
+  const LearnedCoord m = coord_of(*model);
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b > B) return;
   size_t lo = 0, hi = n;  // the answer lies in [lo, hi]

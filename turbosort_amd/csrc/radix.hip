@@ -126,7 +126,7 @@ __device__ __forceinline__ void load_knots(unsigned long long* lds, const Learne
 }
 
 template <class U>
-__device__ __forceinline__ uint32_t bucket_of(U k, const unsigned long long* knots, const LearnedModel& m,
+__device__ __forceinline__ uint32_t bucket_of(U k, const unsigned long long* knots, const LearnedCoord& m,
                                               uint32_t shift) {
   return learned_cdf_pairs(learned_coord((unsigned long long)k, m), knots) >> shift;
 }
@@ -199,7 +199,11 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
   __shared__ uint32_t h[P * kSegments * 256];
   __shared__ unsigned long long knots[LEARNED ? kKnots : 1];
   const unsigned tid = threadIdx.x, lane = tid & 63;
-  if (LEARNED) load_knots(knots, model);
+  LearnedCoord mc{};
+  if constexpr (LEARNED) {
+    mc = coord_of(*model);
+    load_knots(knots, model);
+  }
   for (unsigned i = tid; i < P * kSegments * 256; i += kHistThreads) h[i] = 0;
   {
     const size_t g = size_t(blockIdx.x) * kHistThreads + tid, gs = size_t(gridDim.x) * kHistThreads;
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(HistUnroll<U>::threads) void k_hist(const U* __rest
 #pragma unroll
     for (int r = 0; r < UN; ++r) {
       if constexpr (LEARNED)
-        dk[r] = bucket_of(k[r], knots, *model, model->bucket_shift);
+        dk[r] = bucket_of(k[r], knots, mc, mc.bucket_shift);
       else
         dk[r] = k[r];
     }
@@ -574,8 +578,9 @@ template <class U, bool VALS, int IPT, bool LEARNED = false, bool SMALL = false,
 template <class U, bool VALS, class S, bool FULL, bool LEARNED, bool SMALL, int RW>
 __device__ __forceinline__ void run_tile(
     SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt, LEARNED, SMALL, RW>& L,
-    const SweepArgs<U>& a, S* __restrict__ status, int pass, unsigned shift, bool first, bool last, size_t tile_base,
-    uint32_t valid, size_t first_slot, size_t slot, bool first_round, unsigned long long digit_base) {
+    const SweepArgs<U>& a, const LearnedCoord& mc, S* __restrict__ status, int pass, unsigned shift, bool first,
+    bool last, size_t tile_base, uint32_t valid, size_t first_slot, size_t slot, bool first_round,
+    unsigned long long digit_base) {
   using R = typename RecOf<RW>::T;
   constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt;
   constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads;
@@ -632,7 +637,7 @@ __device__ __forceinline__ void run_tile(
   uint32_t bk[LEARNED ? IPT : 1];
   if constexpr (LEARNED) {
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) bk[j] = bucket_of(k[j], L.knots, *a.model, a.model->bucket_shift);
+    for (int j = 0; j < IPT; ++j) bk[j] = bucket_of(k[j], L.knots, mc, mc.bucket_shift);
   }
   auto dig = [&](int j) {
     if constexpr (LEARNED)
@@ -764,7 +769,7 @@ __device__ __forceinline__ void run_tile(
         const U kk = L.keys[s];
         // Learned digits are evaluated again rather than staged: the pass
         // is far from VALU-bound, and the LDS goes to a larger tile.
-        const uint32_t d = LEARNED ? digit_of(bucket_of(kk, L.knots, *a.model, a.model->bucket_shift), shift)
+        const uint32_t d = LEARNED ? digit_of(bucket_of(kk, L.knots, mc, mc.bucket_shift), shift)
                                    : digit_of(kk, shift);
         const size_t o = size_t(L.gofs[d]) + s;
         if (keys_out) dst[o] = transform(kk);
@@ -853,7 +858,11 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
   uint32_t xcc;
   asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
 
-  if (LEARNED) load_knots(L.knots, a.model);
+  LearnedCoord mc{};
+  if constexpr (LEARNED) {
+    mc = coord_of(*a.model);
+    load_knots(L.knots, a.model);
+  }
   // Rows of the next pass's look-back words: a claimed tile clears its
   // slot's, and the rows past this pass's tiles are cleared here.
   S* next_row = status + size_t((pass + 1) & 1) * a.num_tiles * 256;
@@ -897,10 +906,10 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
     // in this segment.
     const unsigned long long digit_base = tid < 256 ? meta->pbase[pass][seg][tid] : 0ull;
     if (valid == uint32_t(TILE))
-      run_tile<U, VALS, S, true, LEARNED, SMALL, RW>(L, a, status, pass, shift, first, last, tile_base, valid,
+      run_tile<U, VALS, S, true, LEARNED, SMALL, RW>(L, a, mc, status, pass, shift, first, last, tile_base, valid,
                                                        first_slot, slot, round == 0, digit_base);
     else
-      run_tile<U, VALS, S, false, LEARNED, SMALL, RW>(L, a, status, pass, shift, first, last, tile_base, valid,
+      run_tile<U, VALS, S, false, LEARNED, SMALL, RW>(L, a, mc, status, pass, shift, first, last, tile_base, valid,
                                                         first_slot, slot, round == 0, digit_base);
     if (!PERSIST) return;
     __syncthreads();  // the tile's LDS is reused by the next claim
