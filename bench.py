@@ -469,6 +469,11 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5, host: bool 
     executed = 16 + (36 + 40 * (passes - 1) if passes else 0)
     others = [other_src.clone() for _ in range(steps)]
     dt_apply = timed(lambda i: buf.apply_order(others[i]), steps)
+    # The same order applied to a copy of the 16-byte records: the gather a
+    # by-key sort of records that are not carried would run.
+    recs = [src.clone() for _ in range(steps)]
+    dt_apply16 = timed(lambda i: buf.apply_order(recs[i]), steps)
+    del recs
     # order() is the stable argsort iff it is a permutation along which the
     # scores' radix keys do not decrease and equal keys keep increasing order.
     order = buf.order().view(torch.int32).long()
@@ -484,13 +489,15 @@ def run_by_key(steps: int, ctx, n: int = 100_000_000, seed: int = 5, host: bool 
     del rs, others, order, sbits, radix, rk, seen
     out = dict(desc="stable sort_by_key, 10^8 16-byte records {u32 id, f32 score, u64 payload} keyed on the "
                     "score (N(0,1), 1 % NaN/inf/-0.0, 20 % duplicated), records carried through the key passes; "
-                    "apply_order on u64[10^8]",
+                    "apply_order on u64[10^8] and on a copy of the records",
                n=n, ms_per_step=dt * 1e3, records_per_s=n / dt, algo_bytes_per_record=120,
                algorithmic_GBps=120 * n / dt / 1e9, roofline_frac=120 * n / dt / 1e9 / HBM_PEAK_GBS,
                executed=dict(passes=passes, bytes_per_record=executed,
                              roofline_frac=executed * n / dt / 1e9 / HBM_PEAK_GBS),
                apply_order_ms=dt_apply * 1e3, apply_order_GBps=20 * n / dt_apply / 1e9,
-               apply_order_roofline_frac=20 * n / dt_apply / 1e9 / HBM_PEAK_GBS, ok=ok)
+               apply_order_roofline_frac=20 * n / dt_apply / 1e9 / HBM_PEAK_GBS,
+               apply_order_records_ms=dt_apply16 * 1e3,
+               apply_order_records_roofline_frac=36 * n / dt_apply16 / 1e9 / HBM_PEAK_GBS, ok=ok)
     if host:
         hr = torch.empty((n, 16), dtype=torch.uint8, pin_memory=True)
         hr.copy_(src)

@@ -69,3 +69,20 @@ def test_split_part_after_small_tile_switch():
     assert L.ts_plan_status_bytes(2, SMALL_N + 1, -1) >= part
     # 64-bit look-back words from 2^30 keys on.
     assert L.ts_plan_status_bytes(2, 1 << 30, 0) == 2 * ((1 << 30) // 32768 + 8) * 256 * 8
+
+
+def test_record_kinds_use_their_own_tiles():
+    """Carried 8- / 16-byte records, with or without the order, run tiles of
+    their own (512 lanes x 24 / 14 keys with the order, x 36 / 16 without):
+    each kind's status is two rows of 256 look-back words per tile, plus one
+    partial tile per segment, as wide as n's words."""
+    from turbosort_amd import _native
+
+    L = _native.load()
+    tiles = {3: 512 * 24, 4: 512 * 14, 5: 512 * 36, 6: 512 * 16}
+    for code in CODES.values():
+        for n in sizes():
+            word = 8 if n >= 1 << 30 else 4
+            for kind, tile in tiles.items():
+                want = 2 * ((n + tile - 1) // tile + 8) * 256 * word
+                assert L.ts_plan_status_bytes(code, n, kind) == want, (code, n, kind)
