@@ -416,13 +416,41 @@ template <class U, int T, int IPT, bool VALS> __global__ __launch_bounds__(T) vo
 // lane) is the position order and the pass is stable.  Half the passes of the
 // nibble kernel, for about 40 VALU operations per key and pass.
 
+// 4-byte keys with values are staged as (key, value) pairs, so that a key and
+// its value move with one 8-byte LDS access; other widths use two arrays.
 template <class U, int T, int IPT, bool VALS> struct SegByteLds {
   static constexpr int CAP = T * IPT;
-  alignas(16) U keys[CAP];
-  alignas(16) uint32_t vals[VALS ? CAP : 1];
+  static constexpr bool KV = VALS && sizeof(U) == 4;
+  alignas(16) U keys[KV ? 1 : CAP];
+  alignas(16) uint32_t vals[VALS && !KV ? CAP : 1];
+  alignas(16) uint2 kv[KV ? CAP : 1];
   uint32_t whist[T / 64][256];  // per-wave digit counts, then bases
   uint32_t wsum[4];
   uint32_t any[2];
+  __device__ __forceinline__ void put(uint32_t i, U k, uint32_t v) {
+    if constexpr (KV) {
+      kv[i] = make_uint2(uint32_t(k), v);
+    } else {
+      keys[i] = k;
+      if (VALS) vals[i] = v;
+    }
+  }
+  __device__ __forceinline__ void get(uint32_t i, U& k, uint32_t& v) const {
+    if constexpr (KV) {
+      const uint2 x = kv[i];
+      k = U(x.x);
+      v = x.y;
+    } else {
+      k = keys[i];
+      if (VALS) v = vals[i];
+    }
+  }
+  __device__ __forceinline__ U key(uint32_t i) const {
+    if constexpr (KV)
+      return U(kv[i].x);
+    else
+      return keys[i];
+  }
 };
 
 template <class U, int T, int IPT, bool VALS>
@@ -461,8 +489,7 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
         const uint32_t i = uint32_t(j) * T + tid;
-        L.keys[i] = i < len ? kin[j] : U(~U(0));
-        if (VALS) L.vals[i] = vin[j];
+        L.put(i, i < len ? kin[j] : U(~U(0)), VALS ? vin[j] : 0u);
       }
     }
     if (tid == 0) {
@@ -472,14 +499,13 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
     __syncthreads();
 
     U k[IPT];
-    uint32_t v[VALS ? IPT : 1];
-    const U k0 = L.keys[0];
+    uint32_t v[IPT];
+    const U k0 = L.key(0);
     U vary = 0;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
       const uint32_t p = wb + 64u * j;
-      k[j] = L.keys[p];
-      if (VALS) v[j] = L.vals[p];
+      L.get(p, k[j], v[j]);
       if (p < len) vary |= U(k[j] ^ k0);
     }
     {
@@ -530,25 +556,24 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
       for (int j = 0; j < IPT; ++j) {
         const uint32_t d = digit_of(k[j], shift);
         const uint32_t dst = L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
-        L.keys[dst] = k[j];
-        if (VALS) L.vals[dst] = v[j];
+        L.put(dst, k[j], v[j]);
         if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        k[j] = L.keys[wb + 64u * j];
-        if (VALS) v[j] = L.vals[wb + 64u * j];
-      }
+      for (int j = 0; j < IPT; ++j) L.get(wb + 64u * j, k[j], v[j]);
     }
 
-    // L.keys / L.vals hold the sorted segment; write it out striped.
+    // The LDS holds the sorted segment; write it out striped.
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
       const uint32_t i = uint32_t(j) * T + tid;
       if (i < len) {
-        keys[i] = from_key<U>(L.keys[i], a.mode);
-        if (VALS) vals[i] = L.vals[i];
+        U kk;
+        uint32_t vv;
+        L.get(i, kk, vv);
+        keys[i] = from_key<U>(kk, a.mode);
+        if (VALS) vals[i] = vv;
       }
     }
     __syncthreads();  // the next segment reuses the LDS
