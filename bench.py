@@ -108,14 +108,6 @@ def cpu_sort_baseline(src, threads: int, budget_s: float, what: str):
 
 
 # ------------------------------------------------------------ plain sorts
-def cpu_sample(name: str, n: int):
-    from oracle import closed_form as cf
-
-    _, kind, seed, _, _ = WORKLOADS[name]
-    return (cf.u32_keys(seed, n) if kind == 0 else cf.u64_keys(seed, n) if kind == 1
-            else cf.u64_zero_extended_u32(seed, n))
-
-
 def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx, host: bool = True, cpu=None):
     import torch
 
@@ -196,8 +188,15 @@ def run_workload(name: str, log2n: int, steps: int, warmup: int, ctx, host: bool
     if host:
         out["with_copies"] = run_host(name, log2n, max(3, steps // 3), ctx)
     if cpu is not None:
-        out["cpu_baseline"] = cpu_sort_baseline(cpu_sample(name, 1 << 25), cpu["threads"], cpu["budget"],
-                                                f"first 2^25 keys of the same stream ({desc.split(', ', 1)[1]})")
+        import numpy as np
+
+        # The same keys, copied to the host: the whole workload, not a sample.
+        signed = torch.int32 if w == 4 else torch.int64
+        keys = src.view(signed).cpu().numpy().view(np.uint32 if w == 4 else np.uint64)
+        out["cpu_baseline"] = cpu_sort_baseline(keys, cpu["threads"], cpu["budget"],
+                                                f"the whole workload ({desc.split(', ', 1)[1]})")
+        del keys
+    del src
     return out
 
 
