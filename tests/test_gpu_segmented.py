@@ -347,3 +347,58 @@ def test_msd_shapes_on_structured_segments(gpu, dtype):
         ts.segmented_sort(tk, to_dev(off, gpu), None, max_segment_len=cap, ctx=ctx)
         torch.cuda.synchronize()
         assert np.array_equal(bits(to_host(tk, keys.dtype)), bits(want)), (dtype, c)
+
+
+@pytest.mark.parametrize("dtype", ["u32", "i32", "f32"])
+def test_pair_msd_shapes_on_structured_segments(gpu, dtype):
+    """The key-value MSD shapes (one MSD pass, ranks inside the bins) on
+    segments that take each of their paths: random keys (short bins), keys
+    differing only in their low bits, every key equal (nothing moves), few
+    distinct values and a tight cluster plus one outlier (bins beyond 64 keys:
+    the LSD fallback), and float specials; ties must keep input order."""
+    from turbosort_amd import autotune
+
+    tdt = {"u32": torch.uint32, "i32": torch.int32, "f32": torch.float32}[dtype]
+    npdt = cf.NP_DTYPES[dtype]
+    udt = cf.UNSIGNED_OF[np.dtype(npdt).itemsize]
+    ctx = ts.Context(gpu)
+    g = rng(321)
+    seen = 0
+    for c in range(len(autotune.CLASS_CAPS)):
+        names = autotune.variants(tdt, True, c)
+        shapes = [v for v, nm in enumerate(names) if "ranks in bins" in nm]
+        if not shapes:
+            continue
+        cap, lo = autotune.CLASS_CAPS[c], autotune.CLASS_CAPS[c - 1] + 1
+        parts = []
+        for kind in range(6):
+            m = int(g.integers(lo, cap + 1)) if kind else cap
+            base = random_of(dtype, m, seed=c * 100 + kind + 7)
+            if kind == 1:
+                base = np.sort(base)[:1].repeat(m)
+                base = (base.view(udt) ^ g.integers(0, 1024, m).astype(udt)).view(npdt)
+            elif kind == 2:
+                base = base[:1].repeat(m)
+            elif kind == 3:
+                base = base[g.integers(0, 5, m)]
+            elif kind == 4:
+                u = base.view(udt).copy()
+                u[:] = u[0] ^ g.integers(0, 1 << 16, m).astype(udt)
+                u[0] = ~u[1]
+                base = u.view(npdt)
+            elif kind == 5 and dtype == "f32":
+                base = float_specials_f32(m, seed=c + 50)
+            parts.append(base.astype(npdt))
+        keys = np.concatenate(parts)
+        off = offsets_of([len(p) for p in parts])
+        vals = g.integers(0, 1 << 32, keys.size, dtype=np.uint64).astype(np.uint32)
+        want_k, want_v = cf.segmented_sort(keys, off, vals)
+        for v in shapes:
+            autotune.set_variant(ctx, tdt, True, c, v)
+            tk, tv = to_dev(keys, gpu), to_dev(vals, gpu)
+            ts.segmented_sort(tk, to_dev(off, gpu), tv, max_segment_len=cap, ctx=ctx)
+            torch.cuda.synchronize()
+            assert np.array_equal(bits(to_host(tk, keys.dtype)), bits(want_k)), (dtype, c, names[v])
+            assert np.array_equal(to_host(tv, np.uint32), want_v), (dtype, c, names[v])
+            seen += 1
+    assert seen >= 5

@@ -36,12 +36,12 @@ template <class U> __device__ __forceinline__ void opaque(U& x) {
 }
 
 // Exclusive scan of one value per lane over the first 256 lanes of the
-// workgroup (every lane of the workgroup must call it; s_wave holds one slot
-// per wave).
+// workgroup (every lane of the workgroup must call it; s_wave holds four
+// slots, one per wave of those lanes).
 template <class T> __device__ __forceinline__ T block256_exclusive_add(T v, T* s_wave) {
   const unsigned tid = threadIdx.x, w = tid >> 6;
   T inc = wave_inclusive_add(v);
-  if ((tid & 63) == 63) s_wave[w] = inc;
+  if ((tid & 63) == 63 && w < 4) s_wave[w] = inc;
   __syncthreads();
   T off = 0;
   for (unsigned i = 0; i < w && i < 4; ++i) off += s_wave[i];
@@ -156,6 +156,31 @@ __device__ __forceinline__ void rank_in_wave_atomic(const U (&k)[IPT], uint32_t 
       rk[j >> 1] = r;
   }
 }
+
+// A window of `count` 32-bit words at `base` for range-checked buffer
+// accesses (gfx9 buffer resource, stride 0): a load at a byte offset past the
+// window returns 0 and a store there is dropped by the hardware.  The offset
+// is a VGPR; base and count are wave-uniform, so a lane's address for
+// several windows of one array is one register.
+struct WordWindow {
+  __amdgpu_buffer_rsrc_t r;
+  // base and count must be the same in every lane; they are read from the
+  // first active lane, so the resource sits in scalar registers.
+  __device__ __forceinline__ WordWindow(const void* base, uint32_t count) : r(make(base, count)) {}
+  static __device__ __forceinline__ __amdgpu_buffer_rsrc_t make(const void* base, uint32_t count) {
+    const uint64_t b = reinterpret_cast<uint64_t>(base);
+    const uint64_t ub = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b))))) |
+                        (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b >> 32))))) << 32);
+    const int bytes = __builtin_amdgcn_readfirstlane(int(count * 4u));
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(ub), short(0), bytes, 0x00020000);
+  }
+  __device__ __forceinline__ uint32_t load(uint32_t byte_off) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0);
+  }
+  __device__ __forceinline__ void store(uint32_t x, uint32_t byte_off) const {
+    __builtin_amdgcn_raw_buffer_store_b32(x, r, byte_off, 0, 0);
+  }
+};
 
 // Forward key transform of N keys under one (wave-uniform) mode test.
 template <class U, int N> __device__ __forceinline__ void to_keys(U (&k)[N], int mode) {

@@ -15,7 +15,10 @@
 //     agree.  Each lane holds a run of consecutive keys; counts of earlier
 //     equal digits come from 16 nibble counters packed in one 64-bit
 //     register, and a workgroup scan over (digit, lane) counters gives each
-//     key its stable destination.
+//     key its stable destination.  Other shapes of these classes: byte
+//     digits (k_seg_radix8), one MSD pass for keys only (k_seg_msd), and for
+//     4-byte keys with values one MSD pass with stable ranks inside the bins
+//     (k_seg_pairs_msd, the default there).
 //   * longer segments are left in the device class for the caller, which
 //     sorts them with the device-wide radix sort.
 #include <hip/hip_runtime.h>
@@ -453,12 +456,64 @@ template <class U, int T, int IPT, bool VALS> struct SegByteLds {
   }
 };
 
-template <class U, int T, int IPT, bool VALS>
-__global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
+// The LSD byte passes over the digits set in `varying`, inside LDS.  The keys
+// and values are in registers in the wave-blocked layout (wave w, step j,
+// lane l holds position w*64*IPT + 64 j + l) and are back there after each
+// pass; after the last pass the LDS also holds the sorted segment in order.
+// `L` provides put/get of a (key, value) slot and the per-wave counters.
+template <class U, int T, int IPT, bool VALS, class Lds>
+__device__ __forceinline__ void lds_byte_passes(Lds& L, U (&k)[IPT], uint32_t (&v)[IPT], uint64_t varying,
+                                                int atomic_rank) {
   static_assert(T >= 256 && T % 64 == 0, "the first 256 lanes carry one digit each");
   static_assert(64 * IPT <= 65536, "wave ranks are 16-bit");
   constexpr int NW = T / 64;
   constexpr int P = int(sizeof(U));
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t wb = w * 64 * IPT + lane;
+#pragma unroll 1
+  for (int p = 0; p < P; ++p) {
+    const unsigned shift = 8u * unsigned(p);
+    if (((varying >> shift) & 0xFFu) == 0) continue;  // every key agrees on this digit
+    for (unsigned x = lane; x < 256; x += 64) L.whist[w][x] = 0;  // this wave's row only
+    uint32_t rk[(IPT + 1) / 2];
+    if (atomic_rank)
+      rank_in_wave_atomic<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
+    else
+      rank_in_wave<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) opaque(k[j]);  // recompute digits below rather than keep them
+    __syncthreads();
+    // Digit bases: digits in order, then waves in order.
+    uint32_t cnt = 0;
+    if (tid < 256) {
+#pragma unroll
+      for (int ww = 0; ww < NW; ++ww) cnt += L.whist[ww][tid];
+    }
+    uint32_t run = block256_exclusive_add(cnt, L.wsum);
+    if (tid < 256) {
+#pragma unroll
+      for (int ww = 0; ww < NW; ++ww) {
+        const uint32_t c = L.whist[ww][tid];
+        L.whist[ww][tid] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t d = digit_of(k[j], shift);
+      const uint32_t dst = L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
+      L.put(dst, k[j], v[j]);
+      if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) L.get(wb + 64u * j, k[j], v[j]);
+  }
+}
+
+template <class U, int T, int IPT, bool VALS>
+__global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
   __shared__ SegByteLds<U, T, IPT, VALS> L;
   const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t wb = w * 64 * IPT + lane;  // this lane's slot for step j: wb + 64 j
@@ -522,47 +577,7 @@ __global__ __launch_bounds__(T) void k_seg_radix8(SegArgs a, int cls) {
       __syncthreads();  // every lane has read L.any before the next segment clears it
       continue;
     }
-
-#pragma unroll 1
-    for (int p = 0; p < P; ++p) {
-      const unsigned shift = 8u * unsigned(p);
-      if (((varying >> shift) & 0xFFu) == 0) continue;  // every key agrees on this digit
-      for (unsigned x = lane; x < 256; x += 64) L.whist[w][x] = 0;  // this wave's row only
-      uint32_t rk[(IPT + 1) / 2];
-      if (a.atomic_rank)
-        rank_in_wave_atomic<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
-      else
-        rank_in_wave<U, IPT, true>(k, rk, shift, L.whist[w], 0u, 0u);
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) opaque(k[j]);  // recompute digits below rather than keep them
-      __syncthreads();
-      // Digit bases: digits in order, then waves in order.
-      uint32_t cnt = 0;
-      if (tid < 256) {
-#pragma unroll
-        for (int ww = 0; ww < NW; ++ww) cnt += L.whist[ww][tid];
-      }
-      uint32_t run = block256_exclusive_add(cnt, L.wsum);
-      if (tid < 256) {
-#pragma unroll
-        for (int ww = 0; ww < NW; ++ww) {
-          const uint32_t c = L.whist[ww][tid];
-          L.whist[ww][tid] = run;
-          run += c;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const uint32_t d = digit_of(k[j], shift);
-        const uint32_t dst = L.whist[w][d] + ((rk[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
-        L.put(dst, k[j], v[j]);
-        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) L.get(wb + 64u * j, k[j], v[j]);
-    }
+    lds_byte_passes<U, T, IPT, VALS>(L, k, v, varying, a.atomic_rank);
 
     // The LDS holds the sorted segment; write it out striped.
 #pragma unroll
@@ -819,6 +834,259 @@ __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
 }
 
 // ---------------------------------------------------------------------------
+// 4-byte keys with values by one most-significant-digit pass and a stable
+// rank inside each bin.  The keys, in registers, are counted by the top BB
+// bits of their offset from the segment's smallest key (2^BB bins, at least
+// one per key slot; 16-bit counters, two to an LDS word).  The counter's
+// atomic add gives each key a slot in its bin, where it writes one word: the
+// key's offset bits below the bin over its position in the segment.  The
+// offset has at most 32 - BB bits below the bin and the position at most BB
+// bits, so the word fits 32 bits, and within a bin the words compare as
+// (key, position) pairs do.  A key's final place is therefore its bin's
+// start plus the words of its bin below its own, which is the stable order
+// (by_key.rs:175-235 per segment).  Keys and values then move once, as 8-byte
+// pairs, into LDS in their final order and leave in a striped store.  A
+// segment with a bin of more than kMsdRunMax keys (the rank costs the square
+// of the bin's size) takes the byte-digit LSD passes instead.
+// threadIdx.x through an empty asm: values derived from it are recomputed
+// where they are used rather than kept live across phases.
+__device__ __forceinline__ unsigned fresh_lane() {
+  unsigned t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+template <int T, int IPT, int BB> struct SegPairMsdLds {
+  static constexpr int CAP = T * IPT, BINS = 1 << BB;
+  union {
+    struct {
+      uint32_t word[CAP];      // per slot, in bin order: (offset bits below the bin, position)
+      uint32_t cnt[BINS / 2];  // bin counts, then bin starts (16-bit halves)
+    } r;
+    uint2 kv[CAP];  // (key, value) pairs in their final order; the LSD passes' staging
+  };
+  uint32_t whist[T / 64][256];  // the LSD passes' per-wave digit counts
+  uint32_t wsum[T / 64];
+  uint32_t flags[2];  // bits in which the keys differ, largest bin
+  uint32_t kmin, kmax;
+  __device__ __forceinline__ void put(uint32_t i, uint32_t k, uint32_t v) { kv[i] = make_uint2(k, v); }
+  __device__ __forceinline__ void get(uint32_t i, uint32_t& k, uint32_t& v) const {
+    const uint2 x = kv[i];
+    k = x.x;
+    v = x.y;
+  }
+};
+
+// At least four waves per SIMD (128 VGPRs): two 512-lane or four 256-lane
+// workgroups per CU, as their LDS allows.
+template <int T, int IPT, int BB>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_seg_pairs_msd(SegArgs a, int cls) {
+  using U = uint32_t;
+  using L_t = SegPairMsdLds<T, IPT, BB>;
+  constexpr int CAP = L_t::CAP, BINS = L_t::BINS, PER = BINS / T;
+  constexpr int PB = __builtin_ctz(unsigned(CAP));
+  static_assert((CAP & (CAP - 1)) == 0 && PB <= BB, "positions must fit below the bin's offset bits");
+  static_assert(PER >= 2 && PER % 2 == 0 && BINS % T == 0, "each lane scans whole counter words");
+  static_assert(CAP < 65536 && BB <= 16, "16-bit bin counters and starts");
+  __shared__ L_t L;
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t count = a.counts[cls];
+  const uint32_t* __restrict__ list = a.lists + size_t(cls) * a.n_segments;
+  // The inverse key transform as two uniform masks: key ^ (top | spread),
+  // spread = every bit for a float key whose top bit is clear.
+  const uint32_t inv_top = a.mode != kIdentity ? 0x80000000u : 0u;
+  const uint32_t inv_spread = a.mode == kFloat ? ~0u : 0u;
+
+  for (uint32_t job = blockIdx.x; job < count; job += gridDim.x) {
+    const uint32_t seg = list[job];
+    const size_t lo = a.offsets[seg];
+    const uint32_t len = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(seg_end(a, seg) - lo))));
+    // Range-checked windows over the segment: positions past its end load
+    // 0 (never used) and store nothing.  Positions derived from the lane id
+    // are recomputed in each phase (from a copy the compiler cannot see
+    // through) rather than held in registers across the phases.
+    const WordWindow kw(static_cast<U*>(a.keys) + lo, len), vw(a.vals + lo, len);
+    // Striped, coalesced load: step j holds position j*T + lane id.
+    U k[IPT];
+    {
+      const unsigned t = fresh_lane();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) k[j] = kw.load((uint32_t(j) * T + t) * 4u);
+    }
+    to_keys(k, a.mode);
+    const U first = to_key(static_cast<const U*>(a.keys)[lo], a.mode);  // one reference key for every wave
+    U vary = 0, mn = first, mx = first;
+    {
+      const unsigned t = fresh_lane();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        if (uint32_t(j) * T + t < len) {
+          vary |= k[j] ^ first;
+          mn = min(mn, k[j]);
+          mx = max(mx, k[j]);
+        }
+    }
+    for (unsigned i = tid; i < unsigned(BINS / 2); i += T) L.r.cnt[i] = 0;
+    if (tid == 0) {
+      L.flags[0] = 0;
+      L.flags[1] = 0;
+      L.kmin = first;
+      L.kmax = first;
+    }
+    __syncthreads();
+    {
+      U o = vary, m0 = mn, m1 = mx;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        o |= U(__shfl_xor(o, d, 64));
+        m0 = min(m0, U(__shfl_xor(m0, d, 64)));
+        m1 = max(m1, U(__shfl_xor(m1, d, 64)));
+      }
+      if (lane == 0 && o) {
+        atomicOr(&L.flags[0], o);
+        atomicMin(&L.kmin, m0);
+        atomicMax(&L.kmax, m1);
+      }
+    }
+    __syncthreads();
+    const uint32_t varying = L.flags[0];
+    if (varying == 0) {  // every key equal: the stable order is the input order
+      __syncthreads();
+      continue;
+    }
+    const U kmin = L.kmin;
+    const int bits = 32 - __clz(int(L.kmax - kmin));
+    const unsigned sh = bits > BB ? unsigned(bits - BB) : 0u;
+    const uint32_t low_mask = (1u << sh) - 1u;  // sh <= 32 - BB < 32
+    // Count the bins; each key keeps its slot in its bin (16 bits, two keys
+    // to a register).
+    uint32_t slot[(IPT + 1) / 2];
+    {
+      const unsigned t = fresh_lane();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        uint32_t s = 0;
+        if (uint32_t(j) * T + t < len) s = msd_bump(L.r.cnt, uint32_t(k[j] - kmin) >> sh);
+        if (j & 1)
+          slot[j >> 1] |= s << 16;
+        else
+          slot[j >> 1] = s;
+      }
+    }
+    __syncthreads();
+    // Bin starts (bins in order) and the largest bin; the lane's counter
+    // words are read again after the scan rather than held in registers.
+    {
+      const unsigned t = fresh_lane();
+      uint32_t sum = 0, big = 0;
+#pragma unroll
+      for (int q = 0; q < PER / 2; ++q) {
+        const uint32_t wv = L.r.cnt[t * (PER / 2) + q];
+        sum += (wv & 0xFFFFu) + (wv >> 16);
+        big = max(big, max(wv & 0xFFFFu, wv >> 16));
+      }
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) big = max(big, uint32_t(__shfl_xor(big, d, 64)));
+      if (lane == 0) atomicMax(&L.flags[1], big);
+      uint32_t run = block_exclusive_add<T>(sum, L.wsum);
+#pragma unroll
+      for (int q = 0; q < PER / 2; ++q) {
+        const uint32_t wv = L.r.cnt[t * (PER / 2) + q];
+        const uint32_t s0 = run;
+        run += wv & 0xFFFFu;
+        L.r.cnt[t * (PER / 2) + q] = s0 | (run << 16);
+        run += wv >> 16;
+      }
+    }
+    // The values are needed only for the final move: their loads fly while
+    // the keys are placed and ranked.
+    uint32_t v[IPT];
+    {
+      const unsigned t = fresh_lane();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) v[j] = vw.load((uint32_t(j) * T + t) * 4u);
+    }
+    __syncthreads();
+    if (L.flags[1] <= uint32_t(kMsdRunMax)) {
+      // Scatter the words into their bins.
+      {
+        const unsigned t = fresh_lane();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          const uint32_t p = uint32_t(j) * T + t;
+          if (p < len) {
+            const uint32_t off = k[j] - kmin, d = off >> sh;
+            const uint32_t s = (slot[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            L.r.word[msd_count(L.r.cnt, d) + s] = ((off & low_mask) << PB) | p;
+          }
+        }
+      }
+      __syncthreads();
+      // Each key's final place: its bin's start plus the words below its own
+      // (independent reads; bins hold a few keys).
+      uint32_t to[(IPT + 1) / 2];
+      {
+        const unsigned t = fresh_lane();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          const uint32_t p = uint32_t(j) * T + t;
+          uint32_t r = 0;
+          if (p < len) {
+            const uint32_t off = k[j] - kmin, d = off >> sh;
+            const uint32_t mine = ((off & low_mask) << PB) | p;
+            const uint32_t b = msd_count(L.r.cnt, d);
+            const uint32_t e = d + 1 < uint32_t(BINS) ? msd_count(L.r.cnt, d + 1) : len;
+            r = b;
+            for (uint32_t q = b; q < e; ++q) r += L.r.word[q] < mine ? 1u : 0u;
+          }
+          if (j & 1)
+            to[j >> 1] |= r << 16;
+          else
+            to[j >> 1] = r;
+          __builtin_amdgcn_sched_barrier(0);  // one key's rank at a time: bounded live ranges
+        }
+      }
+      __syncthreads();  // the words and counters are read; the pairs take their place
+      {
+        const unsigned t = fresh_lane();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+          if (uint32_t(j) * T + t < len) L.kv[(to[j >> 1] >> (16 * (j & 1))) & 0xFFFFu] = make_uint2(k[j], v[j]);
+      }
+    } else {
+      // A crowded bin: LSD byte passes over the bytes in which the keys
+      // differ, from the wave-blocked layout; padding slots hold the largest
+      // key and, coming after every real key, stay last.
+      {
+        const unsigned t = fresh_lane();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          const uint32_t p = uint32_t(j) * T + t;
+          L.kv[p] = p < len ? make_uint2(k[j], v[j]) : make_uint2(~0u, 0u);
+        }
+      }
+      __syncthreads();
+      const uint32_t wb = w * 64 * IPT + lane;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) L.get(wb + 64u * j, k[j], v[j]);
+      lds_byte_passes<U, T, IPT, true>(L, k, v, uint64_t(varying), a.atomic_rank);
+    }
+    __syncthreads();
+    {
+      const unsigned t = fresh_lane();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t p = uint32_t(j) * T + t;
+        const uint2 x = L.kv[p];
+        kw.store(x.x ^ (inv_top | (inv_spread & ~uint32_t(int32_t(x.x) >> 31))), p * 4u);
+        vw.store(x.y, p * 4u);
+      }
+    }
+    __syncthreads();  // the next segment reuses the LDS
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Three-way split of long keys-only segments around a pivot key (equal keys
 // are equal bits, so no order among them is observable): keys below the
 // pivot go to the front of the segment, keys above to the back, and the
@@ -996,7 +1264,14 @@ struct ClassShape {
   int threads, ipt;
   bool bytes;  // byte digits (k_seg_radix8) instead of nibbles
   bool msd;    // keys only: one most-significant-digit pass and short sorts (k_seg_msd)
+  int pair_bins;  // > 0: 4-byte keys with values, one MSD pass into 2^pair_bins bins (k_seg_pairs_msd)
 };
+
+constexpr int kMaxVariants = 6;
+
+constexpr ClassShape pairs_msd(int threads, int ipt, int bin_bits) {
+  return ClassShape{0, threads, ipt, true, false, bin_bits};
+}
 
 // Class c, variant v: variant 0 is the default; the others are the
 // alternatives an autotuner may pick (same capacity, another kernel shape).
@@ -1005,28 +1280,38 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
   // 8-byte keys with values: 16384 keys would need 192 KB of LDS.
   constexpr bool wide = sizeof(U) == 8;
   constexpr bool alt = sizeof(U) >= 4;
+  constexpr bool pairs4 = sizeof(U) == 4 && VALS;
   switch (c) {
     case 0: return v == 0 ? ClassShape{16, 0, 0, false} : ClassShape{};
     case 1: return v == 0 ? ClassShape{32, 0, 0, false} : ClassShape{};
     case 2: return v == 0 ? ClassShape{64, 0, 0, false} : ClassShape{};
     case 3: return v == 0 ? ClassShape{0, 64, 4, false} : ClassShape{};
     case 4:
-      return v == 0 ? ClassShape{0, 64, 16, false} : (alt && v == 1 ? ClassShape{0, 256, 4, true} : ClassShape{});
+      if (v == 0) return ClassShape{0, 64, 16, false};
+      if (alt && v == 1) return ClassShape{0, 256, 4, true};
+      if (pairs4 && v == 2) return pairs_msd(256, 4, 10);
+      return ClassShape{};
     case 5:
-      return v == 0 ? ClassShape{0, 128, 16, false} : (alt && v == 1 ? ClassShape{0, 256, 8, true} : ClassShape{});
+      if (v == 0) return ClassShape{0, 128, 16, false};
+      if (alt && v == 1) return ClassShape{0, 256, 8, true};
+      if (pairs4 && v == 2) return pairs_msd(256, 8, 11);
+      return ClassShape{};
     case 6:
-      return v == 0   ? ClassShape{0, 256, 16, true}
-             : !alt   ? ClassShape{}
-             : v == 1 ? ClassShape{0, 256, 16, false}
-             : v == 2 && !VALS ? ClassShape{0, 256, 16, true, true}
-                               : ClassShape{};
+      if (v == 0) return ClassShape{0, 256, 16, true};
+      if (!alt) return ClassShape{};
+      if (v == 1) return ClassShape{0, 256, 16, false};
+      if (v == 2 && !VALS) return ClassShape{0, 256, 16, true, true};
+      if (v == 2 && pairs4) return pairs_msd(256, 16, 12);
+      return ClassShape{};
     case 7:
-      return v == 0   ? ClassShape{0, 512, 16, true}
-             : !alt   ? ClassShape{}
-             : v == 1 ? ClassShape{0, 512, 16, false}
-             : v == 2 ? ClassShape{0, 1024, 8, true}
-             : v == 3 && !VALS ? ClassShape{0, 512, 16, true, true}
-                               : ClassShape{};
+      if (v == 0) return ClassShape{0, 512, 16, true};
+      if (!alt) return ClassShape{};
+      if (v == 1) return ClassShape{0, 512, 16, false};
+      if (v == 2) return ClassShape{0, 1024, 8, true};
+      if (v == 3 && !VALS) return ClassShape{0, 512, 16, true, true};
+      if (v == 3 && pairs4) return pairs_msd(512, 16, 13);
+      if (v == 4 && pairs4) return pairs_msd(512, 16, 14);
+      return ClassShape{};
     case 8:
       // 8-byte keys with values: 11264 pairs (132 KiB) are what one
       // workgroup's LDS holds.
@@ -1034,11 +1319,13 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
       // 8-byte keys without values: the MSD shape only, with 17 keys per
       // lane (17408 keys in 136 KiB), which the other shapes' LDS cannot hold.
       if (wide) return v == 0 ? ClassShape{0, 1024, 17, true, true} : ClassShape{};
-      return v == 0   ? ClassShape{0, 1024, 16, true}
-             : !alt   ? ClassShape{}
-             : v == 1 ? ClassShape{0, 512, 32, false}
-             : v == 2 && !VALS ? ClassShape{0, 1024, 16, true, true}
-                               : ClassShape{};
+      if (v == 0) return ClassShape{0, 1024, 16, true};
+      if (!alt) return ClassShape{};
+      if (v == 1) return ClassShape{0, 512, 32, false};
+      if (v == 2 && !VALS) return ClassShape{0, 1024, 16, true, true};
+      if (v == 2 && pairs4) return pairs_msd(1024, 16, 14);
+      if (v == 3 && pairs4) return pairs_msd(1024, 16, 15);
+      return ClassShape{};
     default: return ClassShape{};
   }
 }
@@ -1050,7 +1337,7 @@ template <class U, bool VALS> constexpr bool class_exists(int c, int v) {
 
 template <class U, bool VALS> constexpr int class_variants(int c) {
   int n = 0;
-  while (n < 4 && class_exists<U, VALS>(c, n)) ++n;
+  while (n < kMaxVariants && class_exists<U, VALS>(c, n)) ++n;
   return n;
 }
 
@@ -1065,6 +1352,15 @@ void launch_shape(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk
   if constexpr (sh.bitonic_w > 0) {
     if (clk) clk->begin(s, "k_seg_bitonic", 0);
     hipLaunchKernelGGL((k_seg_bitonic<U, sh.bitonic_w, VALS>), dim3(num_cus * 8), dim3(256), 0, s, a, C);
+    if (clk) clk->end(s);
+  } else if constexpr (sh.threads > 0 && sh.pair_bins > 0) {
+    constexpr size_t lds = sizeof(SegPairMsdLds<sh.threads, sh.ipt, sh.pair_bins>);
+    constexpr int by_lds = int((160 * 1024) / lds);
+    constexpr int by_threads = 2048 / sh.threads;
+    constexpr int per_cu = std::max(1, std::min(by_lds, by_threads));
+    if (clk) clk->begin(s, "k_seg_pairs_msd", 0);
+    hipLaunchKernelGGL((k_seg_pairs_msd<sh.threads, sh.ipt, sh.pair_bins>), dim3(num_cus * per_cu), dim3(sh.threads),
+                       0, s, a, C);
     if (clk) clk->end(s);
   } else if constexpr (sh.threads > 0 && sh.msd) {
     constexpr size_t lds = sizeof(SegMsdLds<U, sh.threads, sh.ipt>);
@@ -1103,12 +1399,16 @@ void launch_class(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk
     if constexpr (class_exists<U, VALS>(C, 2)) return launch_shape<U, VALS, C, 2>(a, num_cus, s, clk);
   } else if (v == 3) {
     if constexpr (class_exists<U, VALS>(C, 3)) return launch_shape<U, VALS, C, 3>(a, num_cus, s, clk);
+  } else if (v == 4) {
+    if constexpr (class_exists<U, VALS>(C, 4)) return launch_shape<U, VALS, C, 4>(a, num_cus, s, clk);
+  } else if (v == 5) {
+    if constexpr (class_exists<U, VALS>(C, 5)) return launch_shape<U, VALS, C, 5>(a, num_cus, s, clk);
   }
   launch_shape<U, VALS, C, 0>(a, num_cus, s, clk);
 }
 
 template <class U, bool VALS> const char* variant_name(int c, int v) {
-  static char buf[kSegClasses][4][48];
+  static char buf[kSegClasses][kMaxVariants][64];
   if (c < 0 || c >= kSegDevice || v < 0 || v >= class_variants<U, VALS>(c)) return nullptr;
   ClassShape sh{};
   switch (c) {
@@ -1124,6 +1424,9 @@ template <class U, bool VALS> const char* variant_name(int c, int v) {
   }
   if (sh.bitonic_w)
     std::snprintf(buf[c][v], sizeof buf[c][v], "bitonic, %d lanes per segment", sh.bitonic_w);
+  else if (sh.pair_bins)
+    std::snprintf(buf[c][v], sizeof buf[c][v], "one %d-bit MSD pass, ranks in bins, %d lanes x %d keys",
+                  sh.pair_bins, sh.threads, sh.ipt);
   else if (sh.msd)
     std::snprintf(buf[c][v], sizeof buf[c][v], "one %d-bit MSD pass, %d lanes x %d keys", kMsdBits, sh.threads,
                   sh.ipt);
