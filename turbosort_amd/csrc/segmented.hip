@@ -456,6 +456,20 @@ template <class U, int T, int IPT, bool VALS> struct SegByteLds {
   }
 };
 
+// Exclusive scan of one value per lane over all T lanes of the workgroup.
+template <int T> __device__ __forceinline__ uint32_t block_exclusive_add(uint32_t v, uint32_t* s_wave) {
+  const unsigned tid = threadIdx.x, w = tid >> 6;
+  const uint32_t inc = wave_inclusive_add(v);
+  if ((tid & 63) == 63) s_wave[w] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+#pragma unroll
+  for (int i = 0; i < T / 64; ++i)
+    if (uint32_t(i) < w) off += s_wave[i];
+  __syncthreads();
+  return off + inc - v;
+}
+
 // The LSD byte passes over the digits set in `varying`, inside LDS.  The keys
 // and values are in registers in the wave-blocked layout (wave w, step j,
 // lane l holds position w*64*IPT + 64 j + l) and are back there after each
@@ -464,7 +478,7 @@ template <class U, int T, int IPT, bool VALS> struct SegByteLds {
 template <class U, int T, int IPT, bool VALS, class Lds>
 __device__ __forceinline__ void lds_byte_passes(Lds& L, U (&k)[IPT], uint32_t (&v)[IPT], uint64_t varying,
                                                 int atomic_rank) {
-  static_assert(T >= 256 && T % 64 == 0, "the first 256 lanes carry one digit each");
+  static_assert(T % 64 == 0 && (T >= 256 || 256 % T == 0), "lanes share the 256 digits evenly");
   static_assert(64 * IPT <= 65536, "wave ranks are 16-bit");
   constexpr int NW = T / 64;
   constexpr int P = int(sizeof(U));
@@ -484,19 +498,37 @@ __device__ __forceinline__ void lds_byte_passes(Lds& L, U (&k)[IPT], uint32_t (&
     for (int j = 0; j < IPT; ++j) opaque(k[j]);  // recompute digits below rather than keep them
     __syncthreads();
     // Digit bases: digits in order, then waves in order.
-    uint32_t cnt = 0;
-    if (tid < 256) {
+    if constexpr (T >= 256) {
+      uint32_t cnt = 0;
+      if (tid < 256) {
 #pragma unroll
-      for (int ww = 0; ww < NW; ++ww) cnt += L.whist[ww][tid];
-    }
-    uint32_t run = block256_exclusive_add(cnt, L.wsum);
-    if (tid < 256) {
-#pragma unroll
-      for (int ww = 0; ww < NW; ++ww) {
-        const uint32_t c = L.whist[ww][tid];
-        L.whist[ww][tid] = run;
-        run += c;
+        for (int ww = 0; ww < NW; ++ww) cnt += L.whist[ww][tid];
       }
+      uint32_t run = block256_exclusive_add(cnt, L.wsum);
+      if (tid < 256) {
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) {
+          const uint32_t c = L.whist[ww][tid];
+          L.whist[ww][tid] = run;
+          run += c;
+        }
+      }
+    } else {
+      constexpr int DPL = 256 / T;  // consecutive digits per lane
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int i = 0; i < DPL; ++i)
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) cnt += L.whist[ww][tid * DPL + i];
+      uint32_t run = block_exclusive_add<T>(cnt, L.wsum);
+#pragma unroll
+      for (int i = 0; i < DPL; ++i)
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) {
+          const uint32_t c = L.whist[ww][tid * DPL + i];
+          L.whist[ww][tid * DPL + i] = run;
+          run += c;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -632,20 +664,6 @@ template <class U, int T, int IPT> struct SegMsdLds {
   uint32_t flags[3];  // bits in which the keys differ (low, high word), longest run
   U kmin, kmax;
 };
-
-// Exclusive scan of one value per lane over all T lanes of the workgroup.
-template <int T> __device__ __forceinline__ uint32_t block_exclusive_add(uint32_t v, uint32_t* s_wave) {
-  const unsigned tid = threadIdx.x, w = tid >> 6;
-  const uint32_t inc = wave_inclusive_add(v);
-  if ((tid & 63) == 63) s_wave[w] = inc;
-  __syncthreads();
-  uint32_t off = 0;
-#pragma unroll
-  for (int i = 0; i < T / 64; ++i)
-    if (uint32_t(i) < w) off += s_wave[i];
-  __syncthreads();
-  return off + inc - v;
-}
 
 template <class U, int T, int IPT>
 __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
@@ -877,8 +895,27 @@ template <int T, int IPT, int BB> struct SegPairMsdLds {
   }
 };
 
+// The radix key transform of a 4-byte key and its inverse as uniform masks
+// (no per-key branch): key = bits ^ (top | spread(bits)), where spread is
+// every bit for a negative float; the inverse takes spread from the key.
+struct KeyMasks32 {
+  uint32_t top, spread;
+  __device__ __forceinline__ explicit KeyMasks32(int mode)
+      : top(mode != kIdentity ? 0x80000000u : 0u), spread(mode == kFloat ? ~0u : 0u) {}
+  __device__ __forceinline__ uint32_t to(uint32_t b) const { return b ^ (top | (spread & uint32_t(int32_t(b) >> 31))); }
+  __device__ __forceinline__ uint32_t from(uint32_t k) const {
+    return k ^ (top | (spread & ~uint32_t(int32_t(k) >> 31)));
+  }
+};
+
 // At least four waves per SIMD (128 VGPRs): two 512-lane or four 256-lane
-// workgroups per CU, as their LDS allows.
+// workgroups per CU, as their LDS allows.  Each workgroup walks its segments
+// as a pipeline in which every memory wait falls on data that was requested
+// one phase earlier: a segment's keys are requested before the previous
+// segment's stores, its values after its bins are counted, the next
+// segment's bounds before its values.  Loop turns issue the same memory
+// operations in the same order (an empty window stores nothing), so the
+// in-order memory counter lets the keys' wait leave the stores in flight.
 template <int T, int IPT, int BB>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_seg_pairs_msd(SegArgs a, int cls) {
   using U = uint32_t;
@@ -892,197 +929,229 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_s
   const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t count = a.counts[cls];
   const uint32_t* __restrict__ list = a.lists + size_t(cls) * a.n_segments;
-  // The inverse key transform as two uniform masks: key ^ (top | spread),
-  // spread = every bit for a float key whose top bit is clear.
-  const uint32_t inv_top = a.mode != kIdentity ? 0x80000000u : 0u;
-  const uint32_t inv_spread = a.mode == kFloat ? ~0u : 0u;
+  const KeyMasks32 km(a.mode);
+  const uint32_t G = gridDim.x;
+  uint32_t job = blockIdx.x;
+  if (job >= count) return;
+  auto rfl = [](uint32_t x) { return uint32_t(__builtin_amdgcn_readfirstlane(int(x))); };
+  auto seg_hi = [&](uint32_t seg) { return a.ends ? a.ends[seg] : a.offsets[seg + 1]; };
+  // The low word of a segment's end (lengths fit 32 bits): a 32-bit load,
+  // so no half of a loaded register is dead while the load is in flight.
+  auto seg_hi_low = [&](uint32_t seg) {
+    return reinterpret_cast<const uint32_t*>(a.ends ? a.ends + seg : a.offsets + seg + 1)[0];
+  };
+  if (tid == 0) {
+    L.kmin = ~0u;
+    L.kmax = 0;
+    L.flags[0] = 0;
+    L.flags[1] = 0;
+  }
+  // Range-checked windows over a segment: positions past its end load 0
+  // (never used) and store nothing.  Positions derived from the lane id are
+  // recomputed in each phase (from a copy the compiler cannot see through)
+  // rather than held in registers across the phases.
+  size_t cur_lo, prev_lo = 0;
+  uint32_t cur_len, prev_len = 0;  // prev_len 0: nothing to store
+  {
+    const uint32_t seg = rfl(list[job]);
+    cur_lo = size_t(a.offsets[seg]);
+    cur_len = rfl(uint32_t(seg_hi(seg) - cur_lo));
+    cur_lo = (size_t(rfl(uint32_t(cur_lo >> 32))) << 32) | rfl(uint32_t(cur_lo));
+  }
+  U k[IPT];
+  {
+    const WordWindow kw(static_cast<U*>(a.keys) + cur_lo, cur_len);
+    const unsigned t = fresh_lane();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) k[j] = kw.load((uint32_t(j) * T + t) * 4u);
+  }
+  uint32_t nseg_v = list[min(job + G, count - 1)];  // the next segment's id, in flight
 
-  for (uint32_t job = blockIdx.x; job < count; job += gridDim.x) {
-    const uint32_t seg = list[job];
-    const size_t lo = a.offsets[seg];
-    const uint32_t len = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(seg_end(a, seg) - lo))));
-    // Range-checked windows over the segment: positions past its end load
-    // 0 (never used) and store nothing.  Positions derived from the lane id
-    // are recomputed in each phase (from a copy the compiler cannot see
-    // through) rather than held in registers across the phases.
-    const WordWindow kw(static_cast<U*>(a.keys) + lo, len), vw(a.vals + lo, len);
-    // Striped, coalesced load: step j holds position j*T + lane id.
-    U k[IPT];
+  for (;;) {
+    __syncthreads();  // the previous segment's pairs are in LDS in their final order
     {
-      const unsigned t = fresh_lane();
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) k[j] = kw.load((uint32_t(j) * T + t) * 4u);
-    }
-    to_keys(k, a.mode);
-    const U first = to_key(static_cast<const U*>(a.keys)[lo], a.mode);  // one reference key for every wave
-    U vary = 0, mn = first, mx = first;
-    {
-      const unsigned t = fresh_lane();
-#pragma unroll
-      for (int j = 0; j < IPT; ++j)
-        if (uint32_t(j) * T + t < len) {
-          vary |= k[j] ^ first;
-          mn = min(mn, k[j]);
-          mx = max(mx, k[j]);
-        }
-    }
-    for (unsigned i = tid; i < unsigned(BINS / 2); i += T) L.r.cnt[i] = 0;
-    if (tid == 0) {
-      L.flags[0] = 0;
-      L.flags[1] = 0;
-      L.kmin = first;
-      L.kmax = first;
-    }
-    __syncthreads();
-    {
-      U o = vary, m0 = mn, m1 = mx;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        o |= U(__shfl_xor(o, d, 64));
-        m0 = min(m0, U(__shfl_xor(m0, d, 64)));
-        m1 = max(m1, U(__shfl_xor(m1, d, 64)));
-      }
-      if (lane == 0 && o) {
-        atomicOr(&L.flags[0], o);
-        atomicMin(&L.kmin, m0);
-        atomicMax(&L.kmax, m1);
-      }
-    }
-    __syncthreads();
-    const uint32_t varying = L.flags[0];
-    if (varying == 0) {  // every key equal: the stable order is the input order
-      __syncthreads();
-      continue;
-    }
-    const U kmin = L.kmin;
-    const int bits = 32 - __clz(int(L.kmax - kmin));
-    const unsigned sh = bits > BB ? unsigned(bits - BB) : 0u;
-    const uint32_t low_mask = (1u << sh) - 1u;  // sh <= 32 - BB < 32
-    // Count the bins; each key keeps its slot in its bin (16 bits, two keys
-    // to a register).
-    uint32_t slot[(IPT + 1) / 2];
-    {
-      const unsigned t = fresh_lane();
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        uint32_t s = 0;
-        if (uint32_t(j) * T + t < len) s = msd_bump(L.r.cnt, uint32_t(k[j] - kmin) >> sh);
-        if (j & 1)
-          slot[j >> 1] |= s << 16;
-        else
-          slot[j >> 1] = s;
-      }
-    }
-    __syncthreads();
-    // Bin starts (bins in order) and the largest bin; the lane's counter
-    // words are read again after the scan rather than held in registers.
-    {
-      const unsigned t = fresh_lane();
-      uint32_t sum = 0, big = 0;
-#pragma unroll
-      for (int q = 0; q < PER / 2; ++q) {
-        const uint32_t wv = L.r.cnt[t * (PER / 2) + q];
-        sum += (wv & 0xFFFFu) + (wv >> 16);
-        big = max(big, max(wv & 0xFFFFu, wv >> 16));
-      }
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) big = max(big, uint32_t(__shfl_xor(big, d, 64)));
-      if (lane == 0) atomicMax(&L.flags[1], big);
-      uint32_t run = block_exclusive_add<T>(sum, L.wsum);
-#pragma unroll
-      for (int q = 0; q < PER / 2; ++q) {
-        const uint32_t wv = L.r.cnt[t * (PER / 2) + q];
-        const uint32_t s0 = run;
-        run += wv & 0xFFFFu;
-        L.r.cnt[t * (PER / 2) + q] = s0 | (run << 16);
-        run += wv >> 16;
-      }
-    }
-    // The values are needed only for the final move: their loads fly while
-    // the keys are placed and ranked.
-    uint32_t v[IPT];
-    {
-      const unsigned t = fresh_lane();
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) v[j] = vw.load((uint32_t(j) * T + t) * 4u);
-    }
-    __syncthreads();
-    if (L.flags[1] <= uint32_t(kMsdRunMax)) {
-      // Scatter the words into their bins.
-      {
-        const unsigned t = fresh_lane();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-          const uint32_t p = uint32_t(j) * T + t;
-          if (p < len) {
-            const uint32_t off = k[j] - kmin, d = off >> sh;
-            const uint32_t s = (slot[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
-            L.r.word[msd_count(L.r.cnt, d) + s] = ((off & low_mask) << PB) | p;
-          }
-        }
-      }
-      __syncthreads();
-      // Each key's final place: its bin's start plus the words below its own
-      // (independent reads; bins hold a few keys).
-      uint32_t to[(IPT + 1) / 2];
-      {
-        const unsigned t = fresh_lane();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-          const uint32_t p = uint32_t(j) * T + t;
-          uint32_t r = 0;
-          if (p < len) {
-            const uint32_t off = k[j] - kmin, d = off >> sh;
-            const uint32_t mine = ((off & low_mask) << PB) | p;
-            const uint32_t b = msd_count(L.r.cnt, d);
-            const uint32_t e = d + 1 < uint32_t(BINS) ? msd_count(L.r.cnt, d + 1) : len;
-            r = b;
-            for (uint32_t q = b; q < e; ++q) r += L.r.word[q] < mine ? 1u : 0u;
-          }
-          if (j & 1)
-            to[j >> 1] |= r << 16;
-          else
-            to[j >> 1] = r;
-          __builtin_amdgcn_sched_barrier(0);  // one key's rank at a time: bounded live ranges
-        }
-      }
-      __syncthreads();  // the words and counters are read; the pairs take their place
-      {
-        const unsigned t = fresh_lane();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j)
-          if (uint32_t(j) * T + t < len) L.kv[(to[j >> 1] >> (16 * (j & 1))) & 0xFFFFu] = make_uint2(k[j], v[j]);
-      }
-    } else {
-      // A crowded bin: LSD byte passes over the bytes in which the keys
-      // differ, from the wave-blocked layout; padding slots hold the largest
-      // key and, coming after every real key, stay last.
-      {
-        const unsigned t = fresh_lane();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-          const uint32_t p = uint32_t(j) * T + t;
-          L.kv[p] = p < len ? make_uint2(k[j], v[j]) : make_uint2(~0u, 0u);
-        }
-      }
-      __syncthreads();
-      const uint32_t wb = w * 64 * IPT + lane;
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) L.get(wb + 64u * j, k[j], v[j]);
-      lds_byte_passes<U, T, IPT, true>(L, k, v, uint64_t(varying), a.atomic_rank);
-    }
-    __syncthreads();
-    {
+      const WordWindow kw(static_cast<U*>(a.keys) + prev_lo, prev_len), vw(a.vals + prev_lo, prev_len);
       const unsigned t = fresh_lane();
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
         const uint32_t p = uint32_t(j) * T + t;
         const uint2 x = L.kv[p];
-        kw.store(x.x ^ (inv_top | (inv_spread & ~uint32_t(int32_t(x.x) >> 31))), p * 4u);
+        kw.store(km.from(x.x), p * 4u);
         vw.store(x.y, p * 4u);
       }
+      if (tid == 0) {  // every lane has read the previous segment's bounds and flags
+        L.kmin = ~0u;
+        L.kmax = 0;
+        L.flags[0] = 0;
+        L.flags[1] = 0;
+      }
     }
-    __syncthreads();  // the next segment reuses the LDS
+    __syncthreads();  // the LDS is free for this segment
+    if (job >= count) break;
+    const uint32_t len = cur_len;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) k[j] = km.to(k[j]);
+    {
+      const unsigned t = fresh_lane();
+      U mn = ~0u, mx = 0;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        if (uint32_t(j) * T + t < len) {
+          mn = min(mn, k[j]);
+          mx = max(mx, k[j]);
+        }
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        mn = min(mn, U(__shfl_xor(mn, d, 64)));
+        mx = max(mx, U(__shfl_xor(mx, d, 64)));
+      }
+      if (lane == 0) {
+        atomicMin(&L.kmin, mn);
+        atomicMax(&L.kmax, mx);
+      }
+    }
+    for (unsigned i = tid; i < unsigned(BINS / 2); i += T) L.r.cnt[i] = 0;
+    __syncthreads();
+    const U kmin = L.kmin, kmax = L.kmax;
+    // The next segment's bounds, in flight while this segment is sorted.
+    const bool more = job + G < count;
+    const uint32_t nseg = rfl(nseg_v);
+    const unsigned long long nlo_v = a.offsets[nseg];
+    const uint32_t nhi_v = seg_hi_low(nseg);
+    // With every key equal the stable order is the input order: nothing moves.
+    const bool moves = kmin != kmax;
+    if (moves) {
+      const int bits = 32 - __clz(int(kmax - kmin));
+      const unsigned sh = bits > BB ? unsigned(bits - BB) : 0u;
+      const uint32_t low_mask = (1u << sh) - 1u;  // sh <= 32 - BB < 32
+      const WordWindow vw(a.vals + cur_lo, len);
+      // Count the bins.
+      {
+        const unsigned t = fresh_lane();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+          if (uint32_t(j) * T + t < len) msd_bump(L.r.cnt, uint32_t(k[j] - kmin) >> sh);
+      }
+      __syncthreads();
+      // Bin starts (bins in order) and the largest bin; the lane's counter
+      // words are read again after the scan rather than held in registers.
+      {
+        const unsigned t = fresh_lane();
+        uint32_t sum = 0, big = 0;
+#pragma unroll
+        for (int q = 0; q < PER / 2; ++q) {
+          const uint32_t wv = L.r.cnt[t * (PER / 2) + q];
+          sum += (wv & 0xFFFFu) + (wv >> 16);
+          big = max(big, max(wv & 0xFFFFu, wv >> 16));
+        }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) big = max(big, uint32_t(__shfl_xor(big, d, 64)));
+        if (lane == 0) atomicMax(&L.flags[1], big);
+        uint32_t run = block_exclusive_add<T>(sum, L.wsum);
+#pragma unroll
+        for (int q = 0; q < PER / 2; ++q) {
+          const uint32_t wv = L.r.cnt[t * (PER / 2) + q];
+          const uint32_t s0 = run;
+          run += wv & 0xFFFFu;
+          L.r.cnt[t * (PER / 2) + q] = s0 | (run << 16);
+          run += wv >> 16;
+        }
+      }
+      // The values are needed only for the final move: their loads fly
+      // while the keys are placed and ranked.
+      uint32_t v[IPT];
+      {
+        const unsigned t = fresh_lane();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) v[j] = vw.load((uint32_t(j) * T + t) * 4u);
+      }
+      __syncthreads();
+      if (L.flags[1] <= uint32_t(kMsdRunMax)) {
+        // Scatter the words into their bins, each at its bin's next free
+        // slot (any order inside a bin); the counters end as the bins' ends.
+        {
+          const unsigned t = fresh_lane();
+#pragma unroll
+          for (int j = 0; j < IPT; ++j) {
+            const uint32_t p = uint32_t(j) * T + t;
+            if (p < len) {
+              const uint32_t off = k[j] - kmin;
+              L.r.word[msd_bump(L.r.cnt, off >> sh)] = ((off & low_mask) << PB) | p;
+            }
+          }
+        }
+        __syncthreads();
+        // Each key's final place: its bin's start plus the words below its
+        // own (independent reads; bins hold a few keys).
+        uint32_t to[(IPT + 1) / 2];
+        {
+          const unsigned t = fresh_lane();
+#pragma unroll
+          for (int j = 0; j < IPT; ++j) {
+            const uint32_t p = uint32_t(j) * T + t;
+            uint32_t r = 0;
+            if (p < len) {
+              const uint32_t off = k[j] - kmin, d = off >> sh;
+              const uint32_t mine = ((off & low_mask) << PB) | p;
+              const uint32_t b = d ? msd_count(L.r.cnt, d - 1) : 0u, e = msd_count(L.r.cnt, d);
+              r = b;
+              for (uint32_t q = b; q < e; ++q) r += L.r.word[q] < mine ? 1u : 0u;
+            }
+            if (j & 1)
+              to[j >> 1] |= r << 16;
+            else
+              to[j >> 1] = r;
+            __builtin_amdgcn_sched_barrier(0);  // one key's rank at a time: bounded live ranges
+          }
+        }
+        __syncthreads();  // the words and counters are read; the pairs take their place
+        {
+          const unsigned t = fresh_lane();
+#pragma unroll
+          for (int j = 0; j < IPT; ++j)
+            if (uint32_t(j) * T + t < len) L.kv[(to[j >> 1] >> (16 * (j & 1))) & 0xFFFFu] = make_uint2(k[j], v[j]);
+        }
+      } else {
+        // A crowded bin: LSD byte passes over the bytes in which the keys
+        // differ, from the wave-blocked layout; padding slots hold the
+        // largest key and, coming after every real key, stay last.
+        {
+          const unsigned t = fresh_lane();
+          U vary = 0;
+#pragma unroll
+          for (int j = 0; j < IPT; ++j) {
+            const uint32_t p = uint32_t(j) * T + t;
+            if (p < len) vary |= k[j] ^ kmin;
+            L.kv[p] = p < len ? make_uint2(k[j], v[j]) : make_uint2(~0u, 0u);
+          }
+#pragma unroll
+          for (int d = 1; d < 64; d <<= 1) vary |= U(__shfl_xor(vary, d, 64));
+          if (lane == 0) atomicOr(&L.flags[0], vary);
+        }
+        __syncthreads();
+        const uint32_t varying = L.flags[0];
+        const uint32_t wb = w * 64 * IPT + lane;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) L.get(wb + 64u * j, k[j], v[j]);
+        lds_byte_passes<U, T, IPT, true>(L, k, v, uint64_t(varying), a.atomic_rank);
+      }
+    }
+    // Hand over: this segment is stored at the top of the next turn; the
+    // next segment's keys (an empty window past the last) and the id of the
+    // one after it are requested now.
+    prev_lo = cur_lo;
+    prev_len = moves ? len : 0u;
+    {
+      const size_t nlo = (size_t(rfl(uint32_t(nlo_v >> 32))) << 32) | rfl(uint32_t(nlo_v));
+      cur_len = more ? rfl(nhi_v - uint32_t(nlo_v)) : 0u;
+      cur_lo = nlo;
+      const WordWindow nw(static_cast<U*>(a.keys) + cur_lo, cur_len);
+      const unsigned t = fresh_lane();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) k[j] = nw.load((uint32_t(j) * T + t) * 4u);
+    }
+    nseg_v = list[min(job + 2 * G, count - 1)];
+    job += G;
   }
 }
 
