@@ -588,11 +588,12 @@ ts_status ts_ctx_create(ts_ctx** out, int device) {
   // Keys-only 4-byte segments of the largest class: the MSD shape by
   // default (8-byte keys have no other shape for that class).
   c->seg_variant[2][0][8] = 2;
-  // 4-byte keys with values, segments of 257-16384 keys: one MSD pass with
+  // 4-byte keys with values, segments of 65-16384 keys: one MSD pass with
   // ranks inside the bins (the shapes the autotuner picks on MI355X for the
   // log-uniform batch of 10^5 segments; 2^BB bins = the class's capacity).
-  c->seg_variant[2][1][4] = 2;
-  c->seg_variant[2][1][5] = 2;
+  c->seg_variant[2][1][3] = 1;
+  c->seg_variant[2][1][4] = 3;
+  c->seg_variant[2][1][5] = 3;
   c->seg_variant[2][1][6] = 2;
   c->seg_variant[2][1][7] = 3;
   c->seg_variant[2][1][8] = 2;

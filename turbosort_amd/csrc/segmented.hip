@@ -1354,16 +1354,22 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
     case 0: return v == 0 ? ClassShape{16, 0, 0, false} : ClassShape{};
     case 1: return v == 0 ? ClassShape{32, 0, 0, false} : ClassShape{};
     case 2: return v == 0 ? ClassShape{64, 0, 0, false} : ClassShape{};
-    case 3: return v == 0 ? ClassShape{0, 64, 4, false} : ClassShape{};
+    case 3:
+      if (v == 0) return ClassShape{0, 64, 4, false};
+      if (pairs4 && v == 1) return pairs_msd(64, 4, 8);
+      if (pairs4 && v == 2) return pairs_msd(128, 2, 8);
+      return ClassShape{};
     case 4:
       if (v == 0) return ClassShape{0, 64, 16, false};
       if (alt && v == 1) return ClassShape{0, 256, 4, true};
       if (pairs4 && v == 2) return pairs_msd(256, 4, 10);
+      if (pairs4 && v == 3) return pairs_msd(128, 8, 10);
       return ClassShape{};
     case 5:
       if (v == 0) return ClassShape{0, 128, 16, false};
       if (alt && v == 1) return ClassShape{0, 256, 8, true};
       if (pairs4 && v == 2) return pairs_msd(256, 8, 11);
+      if (pairs4 && v == 3) return pairs_msd(128, 16, 11);
       return ClassShape{};
     case 6:
       if (v == 0) return ClassShape{0, 256, 16, true};
