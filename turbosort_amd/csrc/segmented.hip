@@ -74,7 +74,7 @@ __global__ void k_seg_clear(uint32_t* counts) {
 // adds all its segments of one class with one LDS atomic) and reserves its
 // slots in the global lists with one atomic per class.
 constexpr int kClassifyThreads = 256;
-constexpr int kClassifyItems = 8;
+constexpr int kClassifyItems = 2;
 
 __global__ __launch_bounds__(kClassifyThreads) void k_seg_classify(SegArgs a) {
   __shared__ uint32_t lcount[kSegClasses], lbase[kSegClasses];
