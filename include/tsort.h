@@ -150,7 +150,8 @@ size_t ts_segmented_lds_max_len(ts_dtype dtype, int with_values);
  * ts_segmented_variant_name(...).  A new context starts from the shapes
  * measured fastest on MI355X (variant 0 except: for 4-byte keys with values
  * in classes 3 .. 8, one MSD pass into one bin per key slot with stable ranks
- * inside the bins; the MSD shape for 4-byte keys without values in class 8).
+ * inside the bins; the MSD shape for 4-byte keys without values in class 8
+ * and its pipelined form for 8-byte keys without values there).
  * The choice is kept per context and per key width
  * (with / without values); the learned sort's bucket sorts use the
  * keys-only choice. */

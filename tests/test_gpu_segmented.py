@@ -306,10 +306,10 @@ def test_autotuner_picks_a_variant_per_class(gpu):
 
 @pytest.mark.parametrize("dtype", ["u32", "u64", "f64", "i32"])
 def test_msd_shapes_on_structured_segments(gpu, dtype):
-    """The keys-only MSD shape on segments that take each of its paths: short
-    runs (random keys), keys differing only in their low bits, every key equal,
-    few distinct values and a tight cluster plus one outlier (runs beyond 64
-    keys: the LSD fallback)."""
+    """The keys-only MSD shapes (plain and pipelined) on segments that take
+    each of their paths: short runs (random keys), keys differing only in their
+    low bits, every key equal, few distinct values and a tight cluster plus one
+    outlier (runs beyond 64 keys: the LSD fallback)."""
     from turbosort_amd import autotune
 
     tdt = {"u32": torch.uint32, "u64": torch.uint64, "f64": torch.float64, "i32": torch.int32}[dtype]
@@ -320,7 +320,7 @@ def test_msd_shapes_on_structured_segments(gpu, dtype):
     for c in (6, 7, 8):
         names = autotune.variants(tdt, False, c)
         msd = [v for v, nm in enumerate(names) if "MSD" in nm]
-        assert len(msd) == 1, names
+        assert len(msd) == 2, names  # the plain and the pipelined form
         cap, lo = autotune.CLASS_CAPS[c], autotune.CLASS_CAPS[c - 1] + 1
         parts = []
         for kind in range(5):
@@ -342,11 +342,12 @@ def test_msd_shapes_on_structured_segments(gpu, dtype):
         keys = np.concatenate(parts)
         off = offsets_of([len(p) for p in parts])
         want, _ = cf.segmented_sort(keys, off, None)
-        autotune.set_variant(ctx, tdt, False, c, msd[0])
-        tk = to_dev(keys, gpu)
-        ts.segmented_sort(tk, to_dev(off, gpu), None, max_segment_len=cap, ctx=ctx)
-        torch.cuda.synchronize()
-        assert np.array_equal(bits(to_host(tk, keys.dtype)), bits(want)), (dtype, c)
+        for v in msd:
+            autotune.set_variant(ctx, tdt, False, c, v)
+            tk = to_dev(keys, gpu)
+            ts.segmented_sort(tk, to_dev(off, gpu), None, max_segment_len=cap, ctx=ctx)
+            torch.cuda.synchronize()
+            assert np.array_equal(bits(to_host(tk, keys.dtype)), bits(want)), (dtype, c, names[v])
 
 
 @pytest.mark.parametrize("dtype", ["u32", "i32", "f32"])

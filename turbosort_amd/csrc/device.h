@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "keys.h"
 
 namespace ts {
@@ -180,6 +182,45 @@ struct WordWindow {
   __device__ __forceinline__ void store(uint32_t x, uint32_t byte_off) const {
     __builtin_amdgcn_raw_buffer_store_b32(x, r, byte_off, 0, 0);
   }
+};
+
+// The same window for 32- or 64-bit elements (E = uint32_t or uint64_t);
+// offsets are in bytes.
+template <class E> struct ElemWindow {
+  static_assert(sizeof(E) == 4 || sizeof(E) == 8, "32- or 64-bit elements");
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ ElemWindow(const void* base, uint32_t count)
+      : r(WordWindow::make(base, count * uint32_t(sizeof(E) / 4))) {}
+  __device__ __forceinline__ E load(uint32_t byte_off) const {
+    if constexpr (sizeof(E) == 4) {
+      return E(__builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+    } else {
+      return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+    }
+  }
+  __device__ __forceinline__ void store(E x, uint32_t byte_off) const {
+    if constexpr (sizeof(E) == 4) {
+      __builtin_amdgcn_raw_buffer_store_b32(uint32_t(x), r, byte_off, 0, 0);
+    } else {
+      using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), r, byte_off, 0, 0);
+    }
+  }
+};
+
+// The radix key transform and its inverse as two uniform masks (no per-key
+// branch): key = bits ^ (top | spread), where spread is every bit for a
+// negative float (forward: taken from the bits; inverse: from the key).
+template <class U> struct KeyMasks {
+  U top, spread;
+  __device__ __forceinline__ explicit KeyMasks(int mode)
+      : top(mode != kIdentity ? top_bit<U>() : U(0)), spread(mode == kFloat ? U(~U(0)) : U(0)) {}
+  __device__ __forceinline__ U sign_fill(U x) const {
+    using S = typename std::conditional<sizeof(U) == 8, int64_t, int32_t>::type;
+    return U(S(x) >> (sizeof(U) * 8 - 1));
+  }
+  __device__ __forceinline__ U to(U b) const { return U(b ^ (top | (spread & sign_fill(b)))); }
+  __device__ __forceinline__ U from(U k) const { return U(k ^ (top | (spread & U(~sign_fill(k))))); }
 };
 
 // Forward key transform of N keys under one (wave-uniform) mode test.

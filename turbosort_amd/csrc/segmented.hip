@@ -663,6 +663,9 @@ template <class U, int T, int IPT> struct SegMsdLds {
   uint32_t wsum[T / 64];
   uint32_t flags[3];  // bits in which the keys differ (low, high word), longest run
   U kmin, kmax;
+  // Keys-only slots for lds_byte_passes (no values).
+  __device__ __forceinline__ void put(uint32_t i, U k, uint32_t) { keys[i] = k; }
+  __device__ __forceinline__ void get(uint32_t i, U& k, uint32_t&) const { k = keys[i]; }
 };
 
 template <class U, int T, int IPT>
@@ -1156,6 +1159,239 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_s
 }
 
 // ---------------------------------------------------------------------------
+// Keys-only segments by the MSD pass of k_seg_msd, as a pipeline like
+// k_seg_pairs_msd's: the previous segment is stored at the top of each turn,
+// after the next segment's keys were requested, so their latency hides
+// behind the stores; the bounds of the segment after next are requested one
+// phase ahead; the segment's smallest and largest keys need no reference
+// load (every key equal: min = max).
+template <class U, int T, int IPT>
+__global__ __launch_bounds__(T) void k_seg_msd_pipe(SegArgs a, int cls) {
+  static_assert(T >= 64 && T % 64 == 0 && kMsdBins % (2 * T) == 0, "lanes must divide the bins");
+  static_assert(T * IPT < 65536, "16-bit run counters and final slots");
+  constexpr int PER = kMsdBins / T;
+  __shared__ SegMsdLds<U, T, IPT> L;
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t count = a.counts[cls];
+  const uint32_t* __restrict__ list = a.lists + size_t(cls) * a.n_segments;
+  const KeyMasks<U> km(a.mode);
+  const uint32_t G = gridDim.x;
+  uint32_t job = blockIdx.x;
+  if (job >= count) return;
+  auto rfl = [](uint32_t x) { return uint32_t(__builtin_amdgcn_readfirstlane(int(x))); };
+  auto seg_hi_low = [&](uint32_t seg) {
+    return reinterpret_cast<const uint32_t*>(a.ends ? a.ends + seg : a.offsets + seg + 1)[0];
+  };
+  auto rfl64 = [&](unsigned long long x) {
+    return (size_t(rfl(uint32_t(x >> 32))) << 32) | rfl(uint32_t(x));
+  };
+  if (tid == 0) {
+    L.kmin = U(~U(0));
+    L.kmax = U(0);
+    L.flags[0] = L.flags[1] = L.flags[2] = 0;
+  }
+  size_t cur_lo, prev_lo = 0;
+  uint32_t cur_len, prev_len = 0;  // prev_len 0: nothing to store
+  {
+    const uint32_t seg = rfl(list[job]);
+    const unsigned long long lo = a.offsets[seg];
+    cur_len = rfl(seg_hi_low(seg) - uint32_t(lo));
+    cur_lo = rfl64(lo);
+  }
+  U k[IPT];
+  {
+    const ElemWindow<U> kw(static_cast<U*>(a.keys) + cur_lo, cur_len);
+    const unsigned t = fresh_lane();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) k[j] = kw.load((uint32_t(j) * T + t) * uint32_t(sizeof(U)));
+  }
+  uint32_t nseg_v = list[min(job + G, count - 1)];  // the next segment's id, in flight
+
+  for (;;) {
+    __syncthreads();  // the previous segment is sorted in LDS
+    {
+      const ElemWindow<U> kw(static_cast<U*>(a.keys) + prev_lo, prev_len);
+      const unsigned t = fresh_lane();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t p = uint32_t(j) * T + t;
+        kw.store(km.from(L.keys[p]), p * uint32_t(sizeof(U)));
+      }
+      if (tid == 0) {  // every lane has read the previous segment's bounds and flags
+        L.kmin = U(~U(0));
+        L.kmax = U(0);
+        L.flags[0] = L.flags[1] = L.flags[2] = 0;
+      }
+    }
+    __syncthreads();  // the LDS is free for this segment
+    if (job >= count) break;
+    const uint32_t len = cur_len;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) k[j] = km.to(k[j]);
+    {
+      const unsigned t = fresh_lane();
+      U mn = U(~U(0)), mx = U(0);
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        if (uint32_t(j) * T + t < len) {
+          mn = k[j] < mn ? k[j] : mn;
+          mx = k[j] > mx ? k[j] : mx;
+        }
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const U o0 = U(__shfl_xor(mn, d, 64)), o1 = U(__shfl_xor(mx, d, 64));
+        mn = o0 < mn ? o0 : mn;
+        mx = o1 > mx ? o1 : mx;
+      }
+      if (lane == 0) {
+        atomicMin(&L.kmin, mn);
+        atomicMax(&L.kmax, mx);
+      }
+    }
+    for (unsigned i = tid; i < unsigned(kMsdBins / 2); i += T) L.cnt[i] = 0;
+    __syncthreads();
+    const U kmin = L.kmin, kmax = L.kmax;
+    // The next segment's bounds, in flight while this segment is sorted.
+    const bool more = job + G < count;
+    const uint32_t nseg = rfl(nseg_v);
+    const unsigned long long nlo_v = a.offsets[nseg];
+    const uint32_t nhi_v = seg_hi_low(nseg);
+    // Every key equal: the segment is sorted as it is.
+    const bool moves = kmin != kmax;
+    if (moves) {
+      const uint64_t span = uint64_t(U(kmax - kmin));
+      const int bits = 64 - __clzll((long long)span);
+      const unsigned sh = bits > kMsdBits ? unsigned(bits - kMsdBits) : 0u;
+      auto dig = [&](U x) { return uint32_t(uint64_t(U(x - kmin)) >> sh); };
+      {
+        const unsigned t = fresh_lane();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+          if (uint32_t(j) * T + t < len) msd_bump(L.cnt, dig(k[j]));
+      }
+      __syncthreads();
+      // Run starts (bins in order) and the longest run; the lane's counter
+      // words are read again after the scan rather than held in registers.
+      {
+        const unsigned t = fresh_lane();
+        uint32_t sum = 0, big = 0;
+#pragma unroll
+        for (int q = 0; q < PER / 2; ++q) {
+          const uint32_t wv = L.cnt[t * (PER / 2) + q];
+          sum += (wv & 0xFFFFu) + (wv >> 16);
+          big = max(big, max(wv & 0xFFFFu, wv >> 16));
+        }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) big = max(big, uint32_t(__shfl_xor(big, d, 64)));
+        if (lane == 0) atomicMax(&L.flags[2], big);
+        uint32_t run = block_exclusive_add<T>(sum, L.wsum);
+#pragma unroll
+        for (int q = 0; q < PER / 2; ++q) {
+          const uint32_t wv = L.cnt[t * (PER / 2) + q];
+          const uint32_t s0 = run;
+          run += wv & 0xFFFFu;
+          L.cnt[t * (PER / 2) + q] = s0 | (run << 16);
+          run += wv >> 16;
+        }
+      }
+      __syncthreads();
+      if (L.flags[2] <= uint32_t(kMsdRunMax)) {
+        // Scatter: a key's slot is its bin's next free one (any order inside
+        // a bin); the counters end as the bins' ends.
+        {
+          const unsigned t = fresh_lane();
+#pragma unroll
+          for (int j = 0; j < IPT; ++j)
+            if (uint32_t(j) * T + t < len) L.keys[msd_bump(L.cnt, dig(k[j]))] = k[j];
+        }
+        __syncthreads();
+        // Each slot's final place: its bin's start plus the keys of the bin
+        // that are smaller, or equal and in earlier slots; then every key
+        // moves at once.
+        uint32_t to[(IPT + 1) / 2];
+        {
+          const unsigned t = fresh_lane();
+#pragma unroll
+          for (int j = 0; j < IPT; ++j) {
+            const uint32_t p = uint32_t(j) * T + t;
+            uint32_t r = p;
+            if (p < len) {
+              const U x = L.keys[p];
+              const uint32_t d = dig(x);
+              const uint32_t b = d ? msd_count(L.cnt, d - 1) : 0u, e = msd_count(L.cnt, d);
+              r = b;
+              for (uint32_t q = b; q < e; ++q) {
+                const U y = L.keys[q];
+                r += (y < x || (y == x && q < p)) ? 1u : 0u;
+              }
+            }
+            if (j & 1)
+              to[j >> 1] |= r << 16;
+            else
+              to[j >> 1] = r;
+            __builtin_amdgcn_sched_barrier(0);  // one slot's rank at a time: bounded live ranges
+          }
+        }
+        __syncthreads();
+        {
+          const unsigned t = fresh_lane();
+#pragma unroll
+          for (int j = 0; j < IPT; ++j) k[j] = L.keys[uint32_t(j) * T + t];
+        }
+        __syncthreads();
+        {
+          const unsigned t = fresh_lane();
+#pragma unroll
+          for (int j = 0; j < IPT; ++j)
+            if (uint32_t(j) * T + t < len) L.keys[(to[j >> 1] >> (16 * (j & 1))) & 0xFFFFu] = k[j];
+        }
+      } else {
+        // A run longer than kMsdRunMax keys: LSD byte passes over the bytes
+        // in which the keys differ, from the wave-blocked layout; padding
+        // slots hold the largest key and stay last.
+        {
+          const unsigned t = fresh_lane();
+          U vary = 0;
+#pragma unroll
+          for (int j = 0; j < IPT; ++j) {
+            const uint32_t p = uint32_t(j) * T + t;
+            if (p < len) vary |= U(k[j] ^ kmin);
+            L.keys[p] = p < len ? k[j] : U(~U(0));
+          }
+          const uint64_t x = wave_or<uint64_t>(uint64_t(vary));
+          if (lane == 0) {
+            atomicOr(&L.flags[0], uint32_t(x));
+            if (sizeof(U) == 8) atomicOr(&L.flags[1], uint32_t(x >> 32));
+          }
+        }
+        __syncthreads();
+        const uint64_t varying = uint64_t(L.flags[0]) | (uint64_t(L.flags[1]) << 32);
+        const uint32_t wb = w * 64 * IPT + lane;
+        uint32_t none[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) k[j] = L.keys[wb + 64u * j];
+        lds_byte_passes<U, T, IPT, false>(L, k, none, varying, a.atomic_rank);
+      }
+    }
+    // Hand over: this segment is stored at the top of the next turn; the
+    // next segment's keys (an empty window past the last) and the id of the
+    // one after it are requested now.
+    prev_lo = cur_lo;
+    prev_len = moves ? len : 0u;
+    {
+      cur_len = more ? rfl(nhi_v - uint32_t(nlo_v)) : 0u;
+      cur_lo = rfl64(nlo_v);
+      const ElemWindow<U> nw(static_cast<U*>(a.keys) + cur_lo, cur_len);
+      const unsigned t = fresh_lane();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) k[j] = nw.load((uint32_t(j) * T + t) * uint32_t(sizeof(U)));
+    }
+    nseg_v = list[min(job + 2 * G, count - 1)];
+    job += G;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Three-way split of long keys-only segments around a pivot key (equal keys
 // are equal bits, so no order among them is observable): keys below the
 // pivot go to the front of the segment, keys above to the back, and the
@@ -1334,13 +1570,15 @@ struct ClassShape {
   bool bytes;  // byte digits (k_seg_radix8) instead of nibbles
   bool msd;    // keys only: one most-significant-digit pass and short sorts (k_seg_msd)
   int pair_bins;  // > 0: 4-byte keys with values, one MSD pass into 2^pair_bins bins (k_seg_pairs_msd)
+  bool pipe;      // with msd: the pipelined form (k_seg_msd_pipe)
 };
 
 constexpr int kMaxVariants = 6;
 
 constexpr ClassShape pairs_msd(int threads, int ipt, int bin_bits) {
-  return ClassShape{0, threads, ipt, true, false, bin_bits};
+  return ClassShape{0, threads, ipt, true, false, bin_bits, false};
 }
+constexpr ClassShape msd_pipe(int threads, int ipt) { return ClassShape{0, threads, ipt, true, true, 0, true}; }
 
 // Class c, variant v: variant 0 is the default; the others are the
 // alternatives an autotuner may pick (same capacity, another kernel shape).
@@ -1377,6 +1615,7 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
       if (v == 1) return ClassShape{0, 256, 16, false};
       if (v == 2 && !VALS) return ClassShape{0, 256, 16, true, true};
       if (v == 2 && pairs4) return pairs_msd(256, 16, 12);
+      if (v == 3 && !VALS) return msd_pipe(256, 16);
       return ClassShape{};
     case 7:
       if (v == 0) return ClassShape{0, 512, 16, true};
@@ -1386,6 +1625,7 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
       if (v == 3 && !VALS) return ClassShape{0, 512, 16, true, true};
       if (v == 3 && pairs4) return pairs_msd(512, 16, 13);
       if (v == 4 && pairs4) return pairs_msd(512, 16, 14);
+      if (v == 4 && !VALS) return msd_pipe(512, 16);
       return ClassShape{};
     case 8:
       // 8-byte keys with values: 11264 pairs (132 KiB) are what one
@@ -1393,13 +1633,14 @@ template <class U, bool VALS> constexpr ClassShape class_shape(int c, int v = 0)
       if (wide && VALS) return v == 0 ? ClassShape{0, 1024, 11, true} : ClassShape{};
       // 8-byte keys without values: the MSD shape only, with 17 keys per
       // lane (17408 keys in 136 KiB), which the other shapes' LDS cannot hold.
-      if (wide) return v == 0 ? ClassShape{0, 1024, 17, true, true} : ClassShape{};
+      if (wide) return v == 0 ? ClassShape{0, 1024, 17, true, true} : v == 1 ? msd_pipe(1024, 17) : ClassShape{};
       if (v == 0) return ClassShape{0, 1024, 16, true};
       if (!alt) return ClassShape{};
       if (v == 1) return ClassShape{0, 512, 32, false};
       if (v == 2 && !VALS) return ClassShape{0, 1024, 16, true, true};
       if (v == 2 && pairs4) return pairs_msd(1024, 16, 14);
       if (v == 3 && pairs4) return pairs_msd(1024, 16, 15);
+      if (v == 3 && !VALS) return msd_pipe(1024, 16);
       return ClassShape{};
     default: return ClassShape{};
   }
@@ -1436,6 +1677,15 @@ void launch_shape(const SegArgs& a, int num_cus, hipStream_t s, KernelClock* clk
     if (clk) clk->begin(s, "k_seg_pairs_msd", 0);
     hipLaunchKernelGGL((k_seg_pairs_msd<sh.threads, sh.ipt, sh.pair_bins>), dim3(num_cus * per_cu), dim3(sh.threads),
                        0, s, a, C);
+    if (clk) clk->end(s);
+  } else if constexpr (sh.threads > 0 && sh.msd && sh.pipe) {
+    constexpr size_t lds = sizeof(SegMsdLds<U, sh.threads, sh.ipt>);
+    constexpr int by_lds = int((160 * 1024) / lds);
+    constexpr int by_threads = 2048 / sh.threads;
+    constexpr int per_cu = std::max(1, std::min(by_lds, by_threads));
+    if (clk) clk->begin(s, "k_seg_msd_pipe", 0);
+    hipLaunchKernelGGL((k_seg_msd_pipe<U, sh.threads, sh.ipt>), dim3(num_cus * per_cu), dim3(sh.threads), 0, s, a,
+                       C);
     if (clk) clk->end(s);
   } else if constexpr (sh.threads > 0 && sh.msd) {
     constexpr size_t lds = sizeof(SegMsdLds<U, sh.threads, sh.ipt>);
@@ -1503,8 +1753,8 @@ template <class U, bool VALS> const char* variant_name(int c, int v) {
     std::snprintf(buf[c][v], sizeof buf[c][v], "one %d-bit MSD pass, ranks in bins, %d lanes x %d keys",
                   sh.pair_bins, sh.threads, sh.ipt);
   else if (sh.msd)
-    std::snprintf(buf[c][v], sizeof buf[c][v], "one %d-bit MSD pass, %d lanes x %d keys", kMsdBits, sh.threads,
-                  sh.ipt);
+    std::snprintf(buf[c][v], sizeof buf[c][v], "one %d-bit MSD pass%s, %d lanes x %d keys", kMsdBits,
+                  sh.pipe ? ", pipelined" : "", sh.threads, sh.ipt);
   else
     std::snprintf(buf[c][v], sizeof buf[c][v], "%s digits, %d lanes x %d keys", sh.bytes ? "byte" : "4-bit",
                   sh.threads, sh.ipt);
