@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-host", action="store_true", help="skip the rates that include host copies")
     p.add_argument("--learned-n", type=int, default=1_000_000_000)
+    p.add_argument("--no-tune", action="store_true",
+                   help="segmented: keep the context's default kernel shapes instead of autotuning them first")
     return p.parse_args()
 
 
@@ -341,7 +343,8 @@ def timed(fn, steps: int):
     return (time.perf_counter() - t0) / steps
 
 
-def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7, host: bool = True, cpu=None):
+def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7, host: bool = True, cpu=None,
+                  tune: bool = True):
     """Batched segmented key-value sort: 10^5 log-uniform segments in [2, 16384],
     u32 keys uniform (SplitMix64 state seed + 2^32), u32 values = global index."""
     import torch
@@ -350,8 +353,15 @@ def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7, hos
     from turbosort_amd import autotune
 
     dev = torch.device("cuda", 0)
-    # Kernel shape per size class, picked on this GPU before the timed runs.
-    tuning = autotune.tune_segmented(torch.uint32, True, ctx=ctx)
+    # Kernel shape per size class, picked on this GPU before the timed runs
+    # (or the context's defaults: the same kernels in every run, as the
+    # counter passes of a profile need).
+    if tune:
+        tuning = autotune.tune_segmented(torch.uint32, True, ctx=ctx)
+        chosen = {str(c): tuning["names"][c][v] for c, v in tuning["choice"].items()}
+    else:
+        chosen = {str(c): autotune.variants(torch.uint32, True, c)[autotune.get_variant(ctx, torch.uint32, True, c)]
+                  for c in range(len(autotune.CLASS_CAPS))}
     off_np = segment_offsets(seed, n_segments)
     off = torch.from_numpy(off_np).to(dev)
     n = int(off[-1])
@@ -402,7 +412,7 @@ def run_segmented(steps: int, ctx, n_segments: int = 100_000, seed: int = 7, hos
                ms_per_step=dt * 1e3, pairs_per_s=n / dt, algo_bytes_per_pair=16,
                algorithmic_GBps=16 * n / dt / 1e9, roofline_frac=16 * n / dt / 1e9 / HBM_PEAK_GBS,
                graph_replay_ms_median=times[len(times) // 2],
-               autotuned={str(c): tuning["names"][c][v] for c, v in tuning["choice"].items()}, ok=ok)
+               **{"autotuned" if tune else "default_shapes": chosen}, ok=ok)
     if host:
         hk = torch.empty(n, dtype=torch.uint32, pin_memory=True)
         hv = torch.empty(n, dtype=torch.int32, pin_memory=True)
@@ -714,7 +724,7 @@ def main():
     also = {}
     for name in [x for x in a.also.split(",") if x and x != a.workload]:
         if name == "segmented":
-            r = run_segmented(max(3, a.steps // 2), ctx, host=host, cpu=cpu)
+            r = run_segmented(max(3, a.steps // 2), ctx, host=host, cpu=cpu, tune=not a.no_tune)
         elif name == "by_key":
             r = run_by_key(max(3, a.steps // 3), ctx, host=host, cpu=cpu)
         elif name == "learned":

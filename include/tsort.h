@@ -158,6 +158,8 @@ size_t ts_segmented_lds_max_len(ts_dtype dtype, int with_values);
 int ts_segmented_variants(ts_dtype dtype, int with_values, int size_class);
 const char* ts_segmented_variant_name(ts_dtype dtype, int with_values, int size_class, int variant);
 ts_status ts_segmented_set_variant(ts_ctx* ctx, ts_dtype dtype, int with_values, int size_class, int variant);
+/* The variant a context runs for that size class (-1: no such class). */
+int ts_segmented_get_variant(const ts_ctx* ctx, ts_dtype dtype, int with_values, int size_class);
 
 /* Learned partitioner.  No reference counterpart: the output equals
  * ts_sort's (turbosort::sort, lib.rs:99-101) for 4- and 8-byte key types,

@@ -296,6 +296,7 @@ def test_autotuner_picks_a_variant_per_class(gpu):
     assert set(table["choice"]) == {c for c in range(9) if len(autotune.variants(torch.uint32, True, c)) > 1}
     for c, v in table["choice"].items():
         assert table["times_ms"][c][v] == min(table["times_ms"][c])
+        assert autotune.get_variant(ctx, torch.uint32, True, c) == v
     off, keys, vals = cf.segmented_workload(7, 500)
     tk, tv = to_dev(keys, gpu), to_dev(vals, gpu)
     ts.segmented_sort(tk, to_dev(off, gpu), tv, max_segment_len=16384, ctx=ctx)

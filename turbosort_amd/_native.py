@@ -19,7 +19,7 @@ EXPORTS = [
     "ts_sbk_create", "ts_sbk_destroy", "ts_sbk_capacity", "ts_sbk_order", "ts_sbk_sort", "ts_sbk_apply_order",
     "ts_sbk_last_error", "ts_segmented_sort_pairs", "ts_segmented_lds_max_len",
     "ts_learned_cfg_default", "ts_learned_sort", "ts_pwl_fit",
-    "ts_segmented_variants", "ts_segmented_variant_name", "ts_segmented_set_variant",
+    "ts_segmented_variants", "ts_segmented_variant_name", "ts_segmented_set_variant", "ts_segmented_get_variant",
     "ts_ctx_set_ranking", "ts_ctx_ranking", "ts_sort_host", "ts_ctx_set_split_rounds", "ts_ctx_split_rounds",
     "ts_ctx_set_divert_min", "ts_ctx_divert_min", "ts_ctx_check", "ts_ctx_last_sort_info", "ts_plan_status_bytes",
     "ts_verify_sorted", "ts_sort_with_buffer_static", "ts_sort_by_key_unordered",
@@ -102,6 +102,8 @@ def load() -> ctypes.CDLL:
     L.ts_segmented_variant_name.restype = ctypes.c_char_p
     L.ts_segmented_set_variant.argtypes = [vp, i32, i32, i32, i32]
     L.ts_segmented_set_variant.restype = i32
+    L.ts_segmented_get_variant.argtypes = [vp, i32, i32, i32]
+    L.ts_segmented_get_variant.restype = i32
     L.ts_learned_cfg_default.argtypes = [ctypes.POINTER(LearnedCfg)]
     L.ts_learned_cfg_default.restype = None
     L.ts_learned_sort.argtypes = [vp, i32, vp, sz, ctypes.POINTER(LearnedCfg), ctypes.POINTER(LearnedReport), vp]

@@ -34,6 +34,14 @@ def set_variant(ctx: Context, dtype: torch.dtype, with_values: bool, size_class:
     ctx.check(ctx._lib.ts_segmented_set_variant(ctx.handle, _dtype_code(dtype), int(with_values), size_class, variant))
 
 
+def get_variant(ctx: Context, dtype: torch.dtype, with_values: bool, size_class: int) -> int:
+    """The variant ``ctx`` runs for size class ``size_class``."""
+    v = ctx._lib.ts_segmented_get_variant(ctx.handle, _dtype_code(dtype), int(with_values), size_class)
+    if v < 0:
+        raise ValueError(f"no size class {size_class}")
+    return v
+
+
 def _random_keys(n: int, dtype: torch.dtype, seed: int, ctx: Context) -> torch.Tensor:
     width = torch.empty(0, dtype=dtype).element_size()
     bits = torch.empty(n, dtype=torch.uint64, device=ctx.device)
@@ -109,4 +117,4 @@ def load(path: str) -> dict:
         return json.load(f)
 
 
-__all__ = ["CLASS_CAPS", "variants", "set_variant", "tune_segmented", "apply", "save", "load"]
+__all__ = ["CLASS_CAPS", "variants", "set_variant", "get_variant", "tune_segmented", "apply", "save", "load"]

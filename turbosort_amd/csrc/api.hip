@@ -1300,6 +1300,11 @@ ts_status ts_segmented_set_variant(ts_ctx* c, ts_dtype dt, int with_values, int 
   return TS_OK;
 }
 
+int ts_segmented_get_variant(const ts_ctx* c, ts_dtype dt, int with_values, int size_class) {
+  if (!c || !valid_dtype(dt) || ts::seg_variants(ts::dtype_bytes(dt), with_values != 0, size_class) == 0) return -1;
+  return c->seg_variant[width_index(ts::dtype_bytes(dt))][with_values ? 1 : 0][size_class];
+}
+
 size_t ts_segmented_lds_max_len(ts_dtype dt, int with_values) {
   if (!valid_dtype(dt)) return 0;
   return ts::seg_lds_max_len(ts::dtype_bytes(dt), with_values != 0);
