@@ -85,6 +85,32 @@ def test_sort_with_buffer_every_dtype(gpu, name):
         check(random_of(name, n, seed=n), gpu, f"{name} n={n}", buffer_len=n + 5)
 
 
+@pytest.mark.parametrize("name", cf.DTYPES)
+def test_one_workgroup_tier_every_dtype(gpu, name):
+    """Keys-only sorts of 4097 keys up to one workgroup's LDS capacity (16384
+    keys of 4 bytes and narrower, 17408 of 8) run the segment kernel for their
+    length on one segment: both sides of every bound, random keys, few
+    distinct values (crowded bins: the kernels' LSD fallback) and descending
+    keys, through sort and sort_with_buffer, whose buffer holds the segment
+    list when it is 8-byte aligned (an unaligned view falls back to the
+    workspace)."""
+    width = np.dtype(cf.NP_DTYPES[name]).itemsize
+    cap = 17408 if width == 8 else 16384
+    g = rng(width * 31 + len(name))
+    for n in (4097, 5000, 8191, 8192, 8193, 12_345, cap - 1, cap, cap + 1):
+        x = random_of(name, n, seed=n + 11)
+        check(x, gpu, f"{name} n={n}")
+        check(x, gpu, f"{name} n={n} buffer", buffer_len=n)
+        check(x[g.integers(0, 3, n)], gpu, f"{name} n={n} few distinct")
+        check(cf.sort(x)[::-1].copy(), gpu, f"{name} n={n} descending")
+    n = 9000
+    x = random_of(name, n, seed=5)
+    t = to_dev(x, gpu)
+    spare = torch.empty(n + 1, dtype=t.dtype, device=gpu)
+    ts.sort_with_buffer(t, spare[1:])
+    assert np.array_equal(bits(to_host(t, x.dtype)), bits(cf.sort(x))), f"{name} unaligned buffer"
+
+
 def test_sort_with_buffer_too_small_panics(gpu):
     t = torch.zeros(10, dtype=torch.uint32, device=gpu)
     buf = torch.zeros(9, dtype=torch.uint32, device=gpu)
@@ -238,7 +264,7 @@ def test_presorted_shortcuts(gpu, n):
 
 
 @pytest.mark.parametrize("name,n", [("u32", 1_000_003), ("u32", 3_000_017), ("f64", 200_000), ("i16", 70_000),
-                                    ("u64", 3_000)])
+                                    ("u64", 3_000), ("f32", 10_000)])
 def test_sort_replays_from_a_graph(gpu, name, n):
     """sort makes no host call once the workspace has grown, so it can be
     captured in a graph; every replay sorts freshly restored input exactly,

@@ -2,8 +2,10 @@
 // messages, workspace management and tier dispatch.
 //
 // Tiers (dispatch.rs:31-67): n <= 1 is a no-op; up to kSmallMax keys one
-// workgroup sorts in LDS (the reference's tiny/small network tiers); above
-// that the device LSD radix sort runs.  The reference's presorted scan and u8
+// workgroup sorts in LDS (the reference's tiny/small network tiers); up to
+// one workgroup's LDS capacity (16384 keys of 4 bytes, 17408 of 8) keys-only
+// sorts take the segmented sort's kernel for that length on one segment;
+// above that the device LSD radix sort runs.  The reference's presorted scan and u8
 // counting sort are output-invisible shortcuts (presorted.rs:10-14): the
 // radix sort produces the same unique output for those inputs.
 #include <hip/hip_runtime.h>
@@ -254,6 +256,9 @@ __global__ void k_iota(uint32_t* out, size_t n) {
   for (size_t i = g; i < n; i += gs) out[i] = uint32_t(i);
 }
 
+// Index of a key width in ts_ctx::seg_variant.
+int width_index(int kb) { return kb == 1 ? 0 : kb == 2 ? 1 : kb == 4 ? 2 : 3; }
+
 ts_status sort_impl(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, hipStream_t s) {
   const int kb = ts::dtype_bytes(dt), mode = ts::dtype_mode(dt);
   clock_reset(c);
@@ -261,6 +266,20 @@ ts_status sort_impl(ts_ctx* c, ts_dtype dt, void* data, size_t n, void* buf, hip
   if (n <= ts::kSmallMax) {
     hipError_t e = ts::small_sort(kb, mode, data, n, nullptr, 0, 0, nullptr, s, &c->clk);
     return e == hipSuccess ? TS_OK : hip_fail(c, e, "small_sort");
+  }
+  if (n <= ts::seg_lds_max_len(kb, false)) {
+    // Still one workgroup's LDS: the keys-only segment kernel for this length
+    // on a one-segment list (two launches instead of the radix sort's nine),
+    // the list in the caller's buffer when it is 8-byte aligned.
+    void* area = buf;
+    if (!area || reinterpret_cast<uintptr_t>(area) % 8 != 0) {
+      ts_status st = ensure(c, ts::kOneSegmentBytes);
+      if (st != TS_OK) return st;
+      area = c->ws;
+    }
+    hipError_t e = ts::lds_sort(kb, mode, data, n, area, c->seg_variant[width_index(kb)][0], c->atomic_rank,
+                                c->dflags, s, &c->clk);
+    return e == hipSuccess ? TS_OK : hip_fail(c, e, "lds_sort");
   }
   ts_status st = ensure(c, sort_ws(n, kb, buf == nullptr));
   if (st != TS_OK) return st;
@@ -331,8 +350,6 @@ ts_status seg_device_sort(ts_ctx* c, int kb, int mode, unsigned char* keys, uint
   }
   return e == hipSuccess ? TS_OK : hip_fail(c, e, "segment gather");
 }
-
-int width_index(int kb) { return kb == 1 ? 0 : kb == 2 ? 1 : kb == 4 ? 2 : 3; }
 
 void set_variants(const ts_ctx* c, int kb, bool vals, ts::SegJob& j) {
   for (int k = 0; k < ts::kSegClasses; ++k) j.variant[k] = c->seg_variant[width_index(kb)][vals ? 1 : 0][k];

@@ -259,6 +259,13 @@ struct SegJob {
   uint32_t class_mask = ~0u;
 };
 uint32_t seg_lds_max_len(int key_bytes, bool vals);
+// The keys-only sort of kSmallMax < n <= seg_lds_max_len(key_bytes, false)
+// keys inside one workgroup (segmented.hip): the keys-only class kernel for
+// that length (variant[c] picks its shape) on a one-segment class list that
+// a one-wave kernel writes into `area` (kOneSegmentBytes of device memory).
+constexpr size_t kOneSegmentBytes = 256;
+hipError_t lds_sort(int key_bytes, int mode, void* keys, size_t n, void* area, const uint8_t* variant,
+                    bool atomic_rank, uint32_t* error, hipStream_t s, KernelClock* clk);
 // Largest segment of size class c (0: a class this width does not use).
 uint32_t seg_class_cap(int key_bytes, bool vals, int size_class);
 // Kernel shapes a size class can run (variant 0 is the default), and a name

@@ -1798,6 +1798,54 @@ template <class U, bool VALS> hipError_t run_segmented(const SegJob& j, hipStrea
   return hipGetLastError();
 }
 
+// One segment of n keys sorted inside one workgroup by the keys-only class
+// kernel for its length: the plain sort's tier between the one-workgroup
+// network (kSmallMax keys) and the device radix sort.  A one-lane-group
+// kernel writes the class list of that single segment into `area`.
+__global__ void k_one_segment(uint32_t* counts, uint32_t* lists, unsigned long long* offsets, int cls,
+                              unsigned long long n) {
+  const unsigned t = threadIdx.x;
+  if (t < unsigned(kSegClasses + 2)) counts[t] = t == unsigned(cls) ? 1u : 0u;
+  if (t < unsigned(kSegClasses)) lists[t] = 0;
+  if (t == 0) {
+    offsets[0] = 0;
+    offsets[1] = n;
+  }
+}
+
+template <class U>
+hipError_t run_one_segment(int mode, void* keys, size_t n, void* area, const uint8_t* variant, bool atomic_rank,
+                           uint32_t* error, hipStream_t s, KernelClock* clk) {
+  SegArgs a = {};
+  auto* counts = static_cast<uint32_t*>(area);
+  auto* lists = counts + (kSegClasses + 2);
+  auto* offsets = reinterpret_cast<unsigned long long*>(lists + kSegClasses + 2);  // 8-byte aligned
+  a.keys = keys;
+  a.vals = nullptr;
+  a.offsets = offsets;
+  a.n = n;
+  a.n_segments = 1;
+  a.counts = counts;
+  a.lists = lists;
+  a.mode = mode;
+  a.atomic_rank = atomic_rank ? 1 : 0;
+  a.error = error;
+  int cls = -1;
+  for (int c = 0; c < kSegClasses; ++c) {
+    a.caps[c] = c < kSegDevice ? class_cap<U, false>(c) : 0u;
+    if (cls < 0 && c < kSegDevice && n <= a.caps[c]) cls = c;
+  }
+  if (cls < 0 || n <= kSmallMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_one_segment, dim3(1), dim3(64), 0, s, counts, lists, offsets, cls, (unsigned long long)n);
+  switch (cls) {  // one workgroup: the grid of a one-CU device
+    case 6: launch_class<U, false, 6>(a, 1, s, clk, variant[6]); break;
+    case 7: launch_class<U, false, 7>(a, 1, s, clk, variant[7]); break;
+    case 8: launch_class<U, false, 8>(a, 1, s, clk, variant[8]); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <class U, bool VALS> uint32_t max_cap() {
   uint32_t m = 0;
   for (int c = 0; c < kSegDevice; ++c) m = std::max(m, class_cap<U, VALS>(c));
@@ -1862,6 +1910,17 @@ const char* seg_variant_name(int key_bytes, bool vals, int size_class, int varia
     case 4: return vals ? variant_name<uint32_t, true>(size_class, variant) : variant_name<uint32_t, false>(size_class, variant);
     case 8: return vals ? variant_name<uint64_t, true>(size_class, variant) : variant_name<uint64_t, false>(size_class, variant);
     default: return nullptr;
+  }
+}
+
+hipError_t lds_sort(int key_bytes, int mode, void* keys, size_t n, void* area, const uint8_t* variant,
+                    bool atomic_rank, uint32_t* error, hipStream_t s, KernelClock* clk) {
+  switch (key_bytes) {
+    case 1: return run_one_segment<uint8_t>(mode, keys, n, area, variant, atomic_rank, error, s, clk);
+    case 2: return run_one_segment<uint16_t>(mode, keys, n, area, variant, atomic_rank, error, s, clk);
+    case 4: return run_one_segment<uint32_t>(mode, keys, n, area, variant, atomic_rank, error, s, clk);
+    case 8: return run_one_segment<uint64_t>(mode, keys, n, area, variant, atomic_rank, error, s, clk);
+    default: return hipErrorInvalidValue;
   }
 }
 
