@@ -98,6 +98,18 @@ __global__ void k_init(SortMeta* m, int P, int gate) {
 #ifndef TS_HIST_BLOCKS_PER_CU_LEARNED
 #define TS_HIST_BLOCKS_PER_CU_LEARNED 4
 #endif
+// Fewest keys a histogram workgroup takes, down to a floor of workgroups
+// per position segment.  Each workgroup ends by adding its nonzero (pass,
+// segment, digit) counts to the global tables, so sorts of a few million
+// keys spread over four workgroups per CU paid mostly for those adds
+// (MI355X, u32: 2^22 keys 181 -> 143 us, 2^21 128 -> 108 us; 2^18 keys want
+// the floor).  Sorts of 2^24 keys and more, and 8-byte keys, are unchanged.
+#ifndef TS_HIST_MIN_KEYS
+#define TS_HIST_MIN_KEYS 16384
+#endif
+#ifndef TS_HIST_MIN_BLOCKS_PER_SEG
+#define TS_HIST_MIN_BLOCKS_PER_SEG 8
+#endif
 // Measured on MI355X at 2^28 keys: 16 keys per lane per step; 8-byte keys
 // (eight digit tables per workgroup) prefer one 1024-lane workgroup per CU
 // (0.55 ms against 0.65 ms for two of 256 lanes), 4-byte keys four of 256.
@@ -1316,8 +1328,11 @@ template <class U, bool LEARNED = false> hipError_t run_radix(const RadixJob& j,
   constexpr int kHistThreads = HistUnroll<U>::threads;
   const int hist_per_cu = LEARNED ? std::max(1, TS_HIST_BLOCKS_PER_CU_LEARNED * 256 / kHistThreads)
                                   : HistUnroll<U>::blocks_per_cu;
-  const uint32_t blocks_per_seg = uint32_t(std::max(1, j.num_cus * hist_per_cu / kSegments));
   const size_t step = size_t(kHistThreads) * HistUnroll<U>::value;
+  const size_t min_keys = std::max(step, size_t(TS_HIST_MIN_KEYS));
+  const size_t by_keys = std::max<size_t>(TS_HIST_MIN_BLOCKS_PER_SEG, (seg0 + min_keys - 1) / min_keys);
+  const uint32_t blocks_per_seg =
+      uint32_t(std::max<size_t>(1, std::min<size_t>(size_t(j.num_cus) * hist_per_cu / kSegments, by_keys)));
   size_t per_block = (seg0 + blocks_per_seg - 1) / blocks_per_seg;
   per_block = std::max(step, ((per_block + step - 1) / step) * step);
   const uint32_t tiles = sweep_tiles(j.n, KB, LEARNED, vals, rw);
