@@ -584,15 +584,22 @@ template <class U, bool VALS, int IPT, bool LEARNED = false, bool SMALL = false,
   uint32_t ticket;
 };
 
+// A persistent workgroup's claim of its next tile: (segment << 24) | round,
+// or none.
+constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
+
 // Everything a tile does after it knows where it is: load, publish its digit
 // counts, rank, look back, stage, store.  FULL tiles (all but each segment's
-// last) run without any per-key bounds test.
-template <class U, bool VALS, class S, bool FULL, bool LEARNED, bool SMALL, int RW>
+// last) run without any per-key bounds test.  LOOP: the workgroup goes on to
+// another tile afterwards (a persistent grid); for keys it then claims the
+// next round of segment sg0 (nt0 rounds) while its stores go out, and leaves
+// the claim, or kNoClaim, in *next.
+template <class U, bool VALS, class S, bool FULL, bool LEARNED, bool SMALL, int RW, bool LOOP = false>
 __device__ __forceinline__ void run_tile(
     SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt, LEARNED, SMALL, RW>& L,
     const SweepArgs<U>& a, const LearnedCoord& mc, S* __restrict__ status, int pass, unsigned shift, bool first,
     bool last, size_t tile_base, uint32_t valid, size_t first_slot, size_t slot, bool first_round,
-    unsigned long long digit_base) {
+    unsigned long long digit_base, uint32_t sg0 = 0, uint32_t nt0 = 0, uint32_t* next = nullptr) {
   using R = typename RecOf<RW>::T;
   constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt;
   constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads;
@@ -600,7 +607,13 @@ __device__ __forceinline__ void run_tile(
   constexpr int TILE = THREADS * IPT;
   constexpr int VB = Flags<S>::kBits;
   constexpr S kAgg = S(1) << VB, kPre = S(2) << VB, kMask = kAgg - 1;
-  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr bool kEarlyClaim = LOOP && RW == 0;
+  unsigned tid_ = threadIdx.x;
+  // In a loop over tiles the lane index is taken afresh for each tile, so
+  // that the lane's addresses are not hoisted out of the loop and kept live
+  // (scratch spills) across it.
+  if constexpr (LOOP) asm volatile("" : "+v"(tid_));
+  const unsigned tid = tid_, lane = tid & 63, w = tid >> 6;
   const uint32_t lb = w * (64 * IPT) + lane;  // this lane's first key in the tile
   const U* __restrict__ src = (pass & 1) ? a.keys_b : a.keys_a;
   U* __restrict__ dst = (pass & 1) ? a.keys_a : a.keys_b;
@@ -808,6 +821,17 @@ __device__ __forceinline__ void run_tile(
     (void)TILE;
     return;
   }
+  // The next tile's claim, in flight with the stores: claimed after this
+  // tile's look-back, so a tile starts right after its claim and successors
+  // rarely wait for it.  The address goes through a lane register so that the
+  // compiler keeps the atomic one lane's (its wave-combined form reads the
+  // result back at once, waiting for the atomic before the stores).
+  uint32_t claimed = 0;
+  if (kEarlyClaim && tid == 0 && nt0 != 0) {
+    uint32_t z = tid;
+    asm volatile("" : "+v"(z));
+    claimed = atomicAdd(&a.seg_tickets[pass * kSegments + sg0 + z], 1u);
+  }
   const bool write_keys = !(VALS && last && a.final_vals);
   const int out_mode = last ? a.mode_out : int(kIdentity);
   if (!write_keys)
@@ -818,6 +842,7 @@ __device__ __forceinline__ void run_tile(
     store([](U x) { return from_key(x, kFloat); }, true);
   else
     store([](U x) { return x; }, true);
+  if (kEarlyClaim && tid == 0) *next = (nt0 != 0 && claimed < nt0) ? ((sg0 << 24) | claimed) : kNoClaim;
   (void)TILE;
 }
 
@@ -840,9 +865,13 @@ __device__ __forceinline__ void run_tile(
 // pass applies the inverse key transform, and in a by-key sort writes only the
 // order to final_vals.  RW > 0: the by-key sort's 8- or 16-byte records are
 // staged and written instead of keys, each pass reading the key out of them,
-// so that the records end in order without a gather.  PERSIST: a grid smaller than the tile count whose
-// workgroups claim tiles until none is left (the gated second sort, which
-// then costs a few microseconds when it does not run).
+// so that the records end in order without a gather.  PERSIST: a grid
+// smaller than the tile count whose workgroups claim tiles until none is left:
+// the gated second sort, which then costs a few microseconds when it does not
+// run, and the keys-only passes of large sorts, where a workgroup claims its
+// next tile while its stores go out instead of ending and a new one starting
+// (MI355X, 2^28 u32 keys: 2.36 -> 2.24 ms per sort; below about 12 tiles per
+// CU the end of the pass costs more than that saves).
 template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false, bool PERSIST = false, int RW = 0>
 __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads)) void k_sweep(
     SweepArgs<U> a, S* __restrict__ status, int pass) {
@@ -869,6 +898,7 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
   const bool last = uint32_t(pass) + 1 == n_live;
   uint32_t xcc;
   asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+  const uint32_t own = xcc & uint32_t(kSegments - 1);  // the segment claimed first
 
   LearnedCoord mc{};
   if constexpr (LEARNED) {
@@ -885,14 +915,26 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
       next_row[size_t(blockIdx.x) * 256 + tid] = 0;
     }
   }
+  uint32_t next = kNoClaim;  // lane 0: the claim made during the last tile
+  bool first_claim = true;
   while (true) {
     for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
     if (tid < 256) L.tcnt[tid] = 0;
-    if (tid == 0) {
+    if (tid == 0 && next != kNoClaim) {
+      L.ticket = next;
+    } else if (tid == 0) {
       uint32_t got = kSegments, r = 0;
+      // Past a workgroup's first claim its own segment is exhausted: read
+      // every counter at once rather than try each one in turn.
+      uint32_t seen[kSegments];
+#pragma unroll
+      for (int q = 0; q < kSegments; ++q)
+        seen[q] = first_claim ? 0u
+                              : __hip_atomic_load(&a.seg_tickets[pass * kSegments + q], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
       for (uint32_t i = 0; i < uint32_t(kSegments); ++i) {
         const uint32_t sg = (xcc + i) & uint32_t(kSegments - 1);
-        if (nt[sg] == 0) continue;
+        if (nt[sg] == 0 || seen[sg] >= nt[sg]) continue;
         r = atomicAdd(&a.seg_tickets[pass * kSegments + sg], 1u);
         if (r < nt[sg]) {
           got = sg;
@@ -901,6 +943,7 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
       }
       L.ticket = (got << 24) | r;
     }
+    first_claim = false;
     __syncthreads();
     const uint32_t claim = __builtin_amdgcn_readfirstlane(L.ticket);
     const uint32_t seg = claim >> 24;
@@ -918,11 +961,13 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
     // in this segment.
     const unsigned long long digit_base = tid < 256 ? meta->pbase[pass][seg][tid] : 0ull;
     if (valid == uint32_t(TILE))
-      run_tile<U, VALS, S, true, LEARNED, SMALL, RW>(L, a, mc, status, pass, shift, first, last, tile_base, valid,
-                                                       first_slot, slot, round == 0, digit_base);
+      run_tile<U, VALS, S, true, LEARNED, SMALL, RW, PERSIST>(L, a, mc, status, pass, shift, first, last, tile_base,
+                                                                valid, first_slot, slot, round == 0, digit_base, own,
+                                                                nt[own], &next);
     else
-      run_tile<U, VALS, S, false, LEARNED, SMALL, RW>(L, a, mc, status, pass, shift, first, last, tile_base, valid,
-                                                        first_slot, slot, round == 0, digit_base);
+      run_tile<U, VALS, S, false, LEARNED, SMALL, RW, PERSIST>(L, a, mc, status, pass, shift, first, last, tile_base,
+                                                                 valid, first_slot, slot, round == 0, digit_base, own,
+                                                                 nt[own], &next);
     if (!PERSIST) return;
     __syncthreads();  // the tile's LDS is reused by the next claim
   }
@@ -1229,6 +1274,13 @@ size_t first_segment(size_t n, int key_bytes, bool learned, bool vals, int rec_w
 #ifndef TS_SWEEP_PERSIST_FROM
 #define TS_SWEEP_PERSIST_FROM 4
 #endif
+// Keys-only passes of 4- and 8-byte keys (one workgroup per CU) run as
+// persistent grids from this many tiles per CU on (MI355X: u32 2^26 keys, 8
+// tiles per CU, 0.715 -> 0.704 ms; 2^27 1.266 -> 1.213 ms; u64 2^25, 8 tiles
+// per CU, 0.854 -> 0.861 ms, 2^27 2.700 -> 2.648 ms).
+#ifndef TS_SWEEP_PERSIST_TILES_PER_CU
+#define TS_SWEEP_PERSIST_TILES_PER_CU 12
+#endif
 
 // gate: the second sort after a divert (persistent launches for keys-only
 // 8-byte keys, run only when the finish found an out-of-order long run).
@@ -1267,6 +1319,7 @@ hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool g
   a.key_shift = uint32_t(8 * (j.key_offset % 4));
   a.rec_mode = j.mode;
   const double bytes = double(j.n) * (2.0 * (rw ? 4.0 * rw : double(sizeof(U))) + (vals ? 8.0 : 0.0));
+  const unsigned cus = unsigned(std::max(1, j.num_cus));
   for (int p = 0; p < DigitSource<U, LEARNED>::P; ++p) {
     if (clk) clk->begin(s, gate ? "k_sweep_fallback" : rw ? "k_sweep_records" : "k_sweep", bytes);
     if (rw) {
@@ -1295,6 +1348,10 @@ hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool g
     } else if (small_sweep(j.n, false, false)) {
       hipLaunchKernelGGL((k_sweep<U, false, S, false, true>), dim3(tiles),
                          dim3(SweepShape<sizeof(U), false, false, true>::threads), 0, s, a, static_cast<S*>(j.status), p);
+    } else if (sizeof(U) >= 4 && size_t(tiles) >= size_t(cus) * TS_SWEEP_PERSIST_TILES_PER_CU) {
+      if constexpr (sizeof(U) >= 4)
+        hipLaunchKernelGGL((k_sweep<U, false, S, false, false, true>), dim3(cus), dim3(SweepShape<sizeof(U)>::threads),
+                           0, s, a, static_cast<S*>(j.status), p);
     } else {
       hipLaunchKernelGGL((k_sweep<U, false, S>), dim3(tiles), dim3(SweepShape<sizeof(U)>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
