@@ -898,7 +898,6 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
   const bool last = uint32_t(pass) + 1 == n_live;
   uint32_t xcc;
   asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
-  const uint32_t own = xcc & uint32_t(kSegments - 1);  // the segment claimed first
 
   LearnedCoord mc{};
   if constexpr (LEARNED) {
@@ -949,6 +948,9 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
     const uint32_t seg = claim >> 24;
     if (seg >= uint32_t(kSegments)) return;
     const uint32_t round = claim & 0xFFFFFFu;
+    // The next claim is tried on this tile's segment first: the workgroup's
+    // own one, or the one it went on to once that was exhausted.
+    const uint32_t seg_tiles = g->nt[seg];
     if (tid < 256) next_row[(size_t(g->slot[seg]) + round) * 256 + tid] = 0;
 
     const size_t seg_lo = g->start[seg];
@@ -962,12 +964,12 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
     const unsigned long long digit_base = tid < 256 ? meta->pbase[pass][seg][tid] : 0ull;
     if (valid == uint32_t(TILE))
       run_tile<U, VALS, S, true, LEARNED, SMALL, RW, PERSIST>(L, a, mc, status, pass, shift, first, last, tile_base,
-                                                                valid, first_slot, slot, round == 0, digit_base, own,
-                                                                nt[own], &next);
+                                                                valid, first_slot, slot, round == 0, digit_base, seg,
+                                                                seg_tiles, &next);
     else
       run_tile<U, VALS, S, false, LEARNED, SMALL, RW, PERSIST>(L, a, mc, status, pass, shift, first, last, tile_base,
-                                                                 valid, first_slot, slot, round == 0, digit_base, own,
-                                                                 nt[own], &next);
+                                                                 valid, first_slot, slot, round == 0, digit_base, seg,
+                                                                 seg_tiles, &next);
     if (!PERSIST) return;
     __syncthreads();  // the tile's LDS is reused by the next claim
   }
