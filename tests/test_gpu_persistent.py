@@ -1,0 +1,106 @@
+"""Sorts large enough for the persistent scatter passes, key for key.
+
+Keys-only sorts of 4- and 8-byte keys run their scatter passes as persistent
+grids (one workgroup per CU claiming tiles, each claiming its next tile on
+the segment it works on while its stores go out) from 12 tiles per CU on:
+about 1.0e8 4-byte keys or 5.7e7 8-byte keys on 256 CUs.  These sizes are
+too large for the oracle to sort in seconds, so each output is compared with
+torch.sort of the same keys on the device (a keys-only sort has one correct
+output), after a check that the path under test ran (executed passes).
+Cases: uniform u32 at the headline size, i32 and f32 with ragged last tiles
+(the signed and float transforms of the last pass), u32 whose upper three
+bits of every byte are zero (all keys in one value segment from the second
+pass on, so every workgroup but those of one XCD finds its own segment empty
+and takes tiles elsewhere), and skewed u64 (a lognormal mixture: unequal
+segments, workgroups moving between them).
+"""
+from __future__ import annotations
+
+import pytest
+import torch
+
+import turbosort_amd as ts
+
+pytestmark = pytest.mark.gpu
+
+PERSIST_TILES_PER_CU = 12
+
+
+def persistent_size(gpu, tile: int) -> int:
+    cus = torch.cuda.get_device_properties(gpu).multi_processor_count
+    return PERSIST_TILES_PER_CU * cus * tile
+
+
+def as_signed_order(t: torch.Tensor) -> torch.Tensor:
+    """An int64 view of unsigned keys whose signed order is their unsigned order."""
+    if t.dtype == torch.uint32:
+        return t.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    return t.view(torch.int64) ^ (-(1 << 63))
+
+
+def check_against_torch(t: torch.Tensor, ctx: ts.Context, passes: int) -> None:
+    if t.dtype in (torch.uint32, torch.uint64):
+        want = torch.sort(as_signed_order(t)).values
+    else:
+        want = torch.sort(t).values
+    ts.sort(t, ctx=ctx)
+    info = ctx.last_sort_info()
+    got = as_signed_order(t) if t.dtype in (torch.uint32, torch.uint64) else t
+    torch.cuda.synchronize()
+    assert info["executed_passes"] == passes, info
+    assert torch.equal(got, want)
+    ctx.check_device()
+
+
+def test_uniform_u32_headline_size(gpu):
+    c = ts.Context(gpu)
+    n = 1 << 28
+    assert n >= persistent_size(gpu, 32768)
+    g = torch.Generator(device=gpu).manual_seed(11)
+    t = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=gpu, generator=g).view(torch.uint32)
+    check_against_torch(t, c, 4)
+
+
+@pytest.mark.parametrize("dtype", [torch.int32, torch.float32])
+def test_ragged_signed_and_float(gpu, dtype):
+    c = ts.Context(gpu)
+    n = persistent_size(gpu, 32768) + 12_345
+    g = torch.Generator(device=gpu).manual_seed(12)
+    if dtype == torch.int32:
+        t = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=gpu, generator=g)
+    else:
+        # Normal values and infinities, no NaN or zero (torch.sort orders
+        # -0.0 and 0.0 as equal, the radix key does not).
+        t = torch.randn(n, device=gpu, generator=g) * 1e3
+        t[t == 0] = 1.0
+        t[:: 1001] = float("inf")
+        t[5 :: 1003] = float("-inf")
+    check_against_torch(t, c, 4)
+
+
+def test_u32_one_value_segment(gpu):
+    """Every byte below 0x20: from the second pass on all keys are in value
+    segment 0, so the workgroups whose own segment is empty take its tiles."""
+    c = ts.Context(gpu)
+    n = persistent_size(gpu, 32768) + 777
+    g = torch.Generator(device=gpu).manual_seed(13)
+    t = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=gpu, generator=g)
+    t = (t & 0x1F1F1F1F).view(torch.uint32)
+    check_against_torch(t, c, 4)
+
+
+def test_skewed_u64(gpu):
+    """Lognormal mixture (the learned workload's distribution) of 2^26 + 3
+    u64 keys: value segments of very different sizes in every pass."""
+    import bench
+
+    c = ts.Context(gpu)
+    n = (1 << 26) + 3
+    assert n >= persistent_size(gpu, 512 * 36)
+    t = bench.lognormal_keys_dev(n, 21, c, as_f64=False)
+    assert t.dtype == torch.uint64
+    want = torch.sort(as_signed_order(t)).values
+    ts.sort(t, ctx=c)
+    torch.cuda.synchronize()
+    assert torch.equal(as_signed_order(t), want)
+    c.check_device()
