@@ -1079,22 +1079,51 @@ static_assert(kFinishHalo < kFinishTile && kFinishHalo % 32 == 0, "a short run t
 // Bytes of per-tile marks the local finish writes into the status area.
 inline size_t finish_mark_bytes(size_t n) { return ((n + kFinishTile - 1) / kFinishTile) * sizeof(uint32_t); }
 
+// LDS slot of tile position q: one pad word per 16 keys, so that each lane's
+// run of 16 consecutive keys starts 17 words after the previous lane's and
+// the lanes' 8-byte reads spread over the banks.
+__device__ __forceinline__ int finish_slot(int q) { return q + (q >> 4); }
+
+// Four workgroups per CU (the grid is four per CU): at most 128 VGPRs.
 template <class U>
-__global__ __launch_bounds__(kFinishThreads) void k_finish_local(SortMeta* m, U* __restrict__ keys_a,
-                                                                 const U* __restrict__ keys_b, size_t n, int mode,
-                                                                 uint32_t* __restrict__ marks) {
+__global__ __launch_bounds__(kFinishThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_local(
+    SortMeta* m, U* __restrict__ keys_a, const U* __restrict__ keys_b, size_t n, int mode,
+    uint32_t* __restrict__ marks) {
   if (m->skip == 0) return;
   const uint32_t shift = m->finish_shift;
   const bool in_place = (m->n_live & 1u) == 0;
   const U* src = in_place ? keys_a : keys_b;
   U* dst = keys_a;
-  __shared__ U w[kFinishTile];
-  const unsigned tid = threadIdx.x;
+  constexpr int NW = kFinishThreads / 64;
+  __shared__ U w[kFinishTile + kFinishTile / kFinishPer];
+  // Each wave's first and last key, one copy per exchange (two odd rounds,
+  // then the final order check, which needs only the first), so that no
+  // exchange waits for the reads of the one before.
+  __shared__ U edge_first[3][NW], edge_last[2][NW];
   const size_t tiles = (n + kFinishTile - 1) / kFinishTile;
   const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;  // gridDim.x is a multiple of 8
   const size_t stretch = (tiles + 7) / 8;
   const size_t t_end = min(tiles, size_t(xcd + 1) * stretch);
+  // The key after this lane's last one (nf: the next lane's first), and the
+  // key before its first (pl: the previous lane's last), across waves
+  // through LDS; past the tile's ends, keys that never move a key (the
+  // largest and the smallest key).
+  unsigned tid = 0, lane = 0, wave = 0;
+  auto exchange = [&](U first, U last, int x, U& nf, U& pl) {
+    const U f = __shfl_down(first, 1, 64), l = __shfl_up(last, 1, 64);
+    if (lane == 0) edge_first[x][wave] = first;
+    if (lane == 63 && x < 2) edge_last[x][wave] = last;
+    __syncthreads();
+    nf = lane != 63 ? f : (wave + 1 < unsigned(NW) ? edge_first[x][wave + 1] : U(~U(0)));
+    pl = lane != 0 ? l : (wave > 0 && x < 2 ? edge_last[x][wave - 1] : U(0));
+  };
   for (size_t tile = size_t(xcd) * stretch + blockIdx.x / 8; tile < t_end; tile += per_xcd) {
+    // The lane index is taken afresh for each tile, so that the lane's many
+    // LDS and global addresses are not hoisted out of the loop and spilled.
+    tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    lane = tid & 63;
+    wave = tid >> 6;
     const size_t base = tile * kFinishTile;
     const int valid = int(min(n - base, size_t(kFinishTile)));
     const U* __restrict__ b = src + base;  // uniform: one 32-bit offset per load
@@ -1103,47 +1132,62 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_local(SortMeta* m, U*
     for (int r = 0; r < kFinishPer; ++r) reg[r] = b[min(r * kFinishThreads + int(tid), valid - 1)];
     to_keys(reg, mode);
     __syncthreads();  // the previous tile's keys are no longer read
+    // Positions past the tile's last key hold the largest key: they never
+    // move a key of the tile and are never written.
 #pragma unroll
     for (int r = 0; r < kFinishPer; ++r) {
       const int q = r * kFinishThreads + int(tid);
-      if (q < valid) w[q] = reg[r];
+      w[finish_slot(q)] = q < valid ? reg[r] : U(~U(0));
     }
     __syncthreads();
-#pragma unroll 1
+    // Lane t holds positions 16t .. 16t + 15 and runs the odd-even rounds on
+    // them in registers: even rounds pair (2i, 2i + 1) inside the lane; odd
+    // rounds pair (2i + 1, 2i + 2) inside it and its last key with the next
+    // lane's first.
+    U c[kFinishPer];
+#pragma unroll
+    for (int i = 0; i < kFinishPer; ++i) c[i] = w[finish_slot(int(tid) * kFinishPer + i)];
+    auto order = [&](int i) {
+      const U lo = c[i] < c[i + 1] ? c[i] : c[i + 1], hi = c[i] < c[i + 1] ? c[i + 1] : c[i];
+      c[i] = lo;
+      c[i + 1] = hi;
+    };
+#pragma unroll
     for (int round = 0; round < kLocalRounds; ++round) {
+      if ((round & 1) == 0) {
 #pragma unroll
-      for (int r = 0; r < kFinishPer / 2; ++r) {
-        const int p = 2 * (r * kFinishThreads + int(tid)) + (round & 1);
-        if (p + 1 < valid) {
-          const U x = w[p], y = w[p + 1];
-          if (x > y) {
-            w[p] = y;
-            w[p + 1] = x;
-          }
-        }
+        for (int i = 0; i + 1 < kFinishPer; i += 2) order(i);
+      } else {
+        U nf, pl;
+        exchange(c[0], c[kFinishPer - 1], round >> 1, nf, pl);
+#pragma unroll
+        for (int i = 1; i + 1 < kFinishPer; i += 2) order(i);
+        c[kFinishPer - 1] = nf < c[kFinishPer - 1] ? nf : c[kFinishPer - 1];
+        c[0] = pl > c[0] ? pl : c[0];
       }
-      __syncthreads();
     }
-    bool left = false;
+    static_assert(kLocalRounds == 4, "two odd rounds: exchanges 0 and 1");
+    U nf, pl;
+    exchange(c[0], c[kFinishPer - 1], 2, nf, pl);
+    bool left = c[kFinishPer - 1] > nf;
 #pragma unroll
-    for (int r = 0; r < kFinishPer; ++r) {
-      const int q = r * kFinishThreads + int(tid);
-      if (q + 1 < valid && w[q] > w[q + 1]) left = true;
-    }
+    for (int i = 0; i + 1 < kFinishPer; ++i) left |= c[i] > c[i + 1];
     left = __syncthreads_or(left);
     if (tid == 0) {
       // A run crossing into this tile: compare prefixes only (the tile
       // before may be moving keys inside that run right now).
-      const bool cross = base > 0 && (uint64_t(to_key(src[base - 1], mode)) >> shift) == (uint64_t(w[0]) >> shift);
+      const bool cross = base > 0 && (uint64_t(to_key(src[base - 1], mode)) >> shift) == (uint64_t(c[0]) >> shift);
       marks[tile] = (left ? 1u : 0u) | (cross ? 2u : 0u);
     }
+    // Each lane writes its own positions: the keys that differ from the
+    // tile's keys as read (still in LDS), or every key when the input was in
+    // the second buffer.
+    U* __restrict__ d = dst + base + size_t(tid) * kFinishPer;
+    const KeyMasks<U> km(mode);
 #pragma unroll
-    for (int r = 0; r < kFinishPer; ++r) {
-      const int q = r * kFinishThreads + int(tid);
-      if (q < valid) {
-        const U v = w[q];
-        if (!in_place || v != reg[r]) dst[base + size_t(q)] = from_key(v, mode);
-      }
+    for (int i = 0; i < kFinishPer; ++i) {
+      const int q = int(tid) * kFinishPer + i;
+      if (q < valid && (!in_place || c[i] != w[finish_slot(q)])) d[i] = km.from(c[i]);
     }
   }
 }
