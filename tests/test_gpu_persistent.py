@@ -1,9 +1,10 @@
 """Sorts large enough for the persistent scatter passes, key for key.
 
-Keys-only sorts of 4- and 8-byte keys run their scatter passes as persistent
-grids (one workgroup per CU claiming tiles, each claiming its next tile on
-the segment it works on while its stores go out) from 12 tiles per CU on:
-about 1.0e8 4-byte keys or 5.7e7 8-byte keys on 256 CUs.  These sizes are
+Keys-only sorts of 4- and 8-byte keys, and record-carrying by-key sorts, run
+their scatter passes as persistent grids (one workgroup per CU claiming
+tiles, each claiming its next tile on the segment it works on while its
+stores go out) from 12 tiles per CU on: about 1.0e8 4-byte keys, 5.7e7
+8-byte keys or 2.2e7 16-byte records with their positions on 256 CUs.  These sizes are
 too large for the oracle to sort in seconds, so each output is compared with
 torch.sort of the same keys on the device (a keys-only sort has one correct
 output), after a check that the path under test ran (executed passes).
@@ -11,8 +12,9 @@ Cases: uniform u32 at the headline size, i32 and f32 with ragged last tiles
 (the signed and float transforms of the last pass), u32 whose upper three
 bits of every byte are zero (all keys in one value segment from the second
 pass on, so every workgroup but those of one XCD finds its own segment empty
-and takes tiles elsewhere), and skewed u64 (a lognormal mixture: unequal
-segments, workgroups moving between them).
+and takes tiles elsewhere), skewed u64 (a lognormal mixture: unequal
+segments, workgroups moving between them), and 16-byte records carried with
+their positions (the by-key passes), checked against a stable torch.sort.
 """
 from __future__ import annotations
 
@@ -104,3 +106,34 @@ def test_skewed_u64(gpu):
     torch.cuda.synchronize()
     assert torch.equal(as_signed_order(t), want)
     c.check_device()
+
+
+def test_carried_records_with_order(gpu):
+    """The by-key shape at persistent size: 16-byte records {u32 id, f32
+    score, u64 payload} keyed on the score (duplicates, NaN, -0.0, inf),
+    carried through the passes with their positions.  order() must be the
+    stable argsort of the scores' radix keys (torch.sort with stable=True of
+    the same keys), and the records must end in that order."""
+    n = persistent_size(gpu, 512 * 14) + 1001
+    g = torch.Generator(device=gpu).manual_seed(14)
+    ids = torch.arange(n, dtype=torch.int32, device=gpu)
+    score = torch.randn(n, generator=g, device=gpu)
+    score = torch.where(torch.rand(n, generator=g, device=gpu) < 0.2, score.round(), score)
+    score[::997] = float("nan")
+    score[1::1009] = -0.0
+    score[2::1013] = float("inf")
+    payload = torch.randint(-(1 << 62), 1 << 62, (n,), generator=g, device=gpu)
+    rec = torch.empty(n, 16, dtype=torch.uint8, device=gpu)
+    rec[:, 0:4] = ids.view(torch.uint8).reshape(n, 4)
+    rec[:, 4:8] = score.view(torch.uint8).reshape(n, 4)
+    rec[:, 8:16] = payload.view(torch.uint8).reshape(n, 8)
+    src = rec.clone()
+    buf = ts.SortByKeyBuffer()
+    ts.sort_by_key_with_buffer(rec, buf, ts.KeyField(4, torch.float32))
+    order = buf.order().view(torch.int32).long() & 0xFFFFFFFF
+    sbits = src[:, 4:8].contiguous().view(torch.int32).reshape(-1).long() & 0xFFFFFFFF
+    radix = torch.where(sbits >= 0x80000000, sbits ^ 0xFFFFFFFF, sbits | 0x80000000)
+    want = torch.sort(radix, stable=True).indices
+    torch.cuda.synchronize()
+    assert torch.equal(order, want)
+    assert torch.equal(rec, src[want])

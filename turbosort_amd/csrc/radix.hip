@@ -591,9 +591,9 @@ constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
 // Everything a tile does after it knows where it is: load, publish its digit
 // counts, rank, look back, stage, store.  FULL tiles (all but each segment's
 // last) run without any per-key bounds test.  LOOP: the workgroup goes on to
-// another tile afterwards (a persistent grid); for keys it then claims the
-// next round of segment sg0 (nt0 rounds) while its stores go out, and leaves
-// the claim, or kNoClaim, in *next.
+// another tile afterwards (a persistent grid); it then claims the next round
+// of segment sg0 (nt0 rounds) while its stores go out, and leaves the claim,
+// or kNoClaim, in *next.
 template <class U, bool VALS, class S, bool FULL, bool LEARNED, bool SMALL, int RW, bool LOOP = false>
 __device__ __forceinline__ void run_tile(
     SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt, LEARNED, SMALL, RW>& L,
@@ -607,7 +607,7 @@ __device__ __forceinline__ void run_tile(
   constexpr int TILE = THREADS * IPT;
   constexpr int VB = Flags<S>::kBits;
   constexpr S kAgg = S(1) << VB, kPre = S(2) << VB, kMask = kAgg - 1;
-  constexpr bool kEarlyClaim = LOOP && RW == 0;
+  constexpr bool kEarlyClaim = LOOP;
   unsigned tid_ = threadIdx.x;
   // In a loop over tiles the lane index is taken afresh for each tile, so
   // that the lane's addresses are not hoisted out of the loop and kept live
@@ -803,6 +803,17 @@ __device__ __forceinline__ void run_tile(
       if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
   };
+  // The next tile's claim, in flight with the stores: claimed after this
+  // tile's look-back, so a tile starts right after its claim and successors
+  // rarely wait for it.  The address goes through a lane register so that the
+  // compiler keeps the atomic one lane's (its wave-combined form reads the
+  // result back at once, waiting for the atomic before the stores).
+  uint32_t claimed = 0;
+  if (kEarlyClaim && tid == 0 && nt0 != 0) {
+    uint32_t z = tid;
+    asm volatile("" : "+v"(z));
+    claimed = atomicAdd(&a.seg_tickets[pass * kSegments + sg0 + z], 1u);
+  }
   if constexpr (RW > 0) {
     // Records leave as they came; the key is read out of each again.
     R* __restrict__ rdst = static_cast<R*>((pass & 1) ? a.recs_a : a.recs_b);
@@ -818,19 +829,9 @@ __device__ __forceinline__ void run_tile(
       }
       if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
+    if (kEarlyClaim && tid == 0) *next = (nt0 != 0 && claimed < nt0) ? ((sg0 << 24) | claimed) : kNoClaim;
     (void)TILE;
     return;
-  }
-  // The next tile's claim, in flight with the stores: claimed after this
-  // tile's look-back, so a tile starts right after its claim and successors
-  // rarely wait for it.  The address goes through a lane register so that the
-  // compiler keeps the atomic one lane's (its wave-combined form reads the
-  // result back at once, waiting for the atomic before the stores).
-  uint32_t claimed = 0;
-  if (kEarlyClaim && tid == 0 && nt0 != 0) {
-    uint32_t z = tid;
-    asm volatile("" : "+v"(z));
-    claimed = atomicAdd(&a.seg_tickets[pass * kSegments + sg0 + z], 1u);
   }
   const bool write_keys = !(VALS && last && a.final_vals);
   const int out_mode = last ? a.mode_out : int(kIdentity);
@@ -868,10 +869,11 @@ __device__ __forceinline__ void run_tile(
 // so that the records end in order without a gather.  PERSIST: a grid
 // smaller than the tile count whose workgroups claim tiles until none is left:
 // the gated second sort, which then costs a few microseconds when it does not
-// run, and the keys-only passes of large sorts, where a workgroup claims its
-// next tile while its stores go out instead of ending and a new one starting
-// (MI355X, 2^28 u32 keys: 2.36 -> 2.24 ms per sort; below about 12 tiles per
-// CU the end of the pass costs more than that saves).
+// run, and the keys-only and record-carrying passes of large sorts, where a
+// workgroup claims its next tile while its stores go out instead of ending
+// and a new one starting (MI355X, 2^28 u32 keys: 2.36 -> 2.27 ms per sort;
+// 10^8 16-byte records by key 3.56 -> 3.53 ms; below about 12 tiles per CU
+// the end of the pass costs more than that saves).
 template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false, bool PERSIST = false, int RW = 0>
 __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads)) void k_sweep(
     SweepArgs<U> a, S* __restrict__ status, int pass) {
@@ -1332,9 +1334,13 @@ size_t first_segment(size_t n, int key_bytes, bool learned, bool vals, int rec_w
 // 8-byte keys, run only when the finish found an out-of-order long run).
 // One pass of a record-carrying sort (RW words per record).
 template <class U, class S, bool VALS, int RW>
-void launch_rec_sweep(uint32_t tiles, hipStream_t s, const SweepArgs<U>& a, void* status, int p) {
-  hipLaunchKernelGGL((k_sweep<U, VALS, S, false, false, false, RW>), dim3(tiles),
-                     dim3(SweepShape<sizeof(U), false, VALS, false, RW>::threads), 0, s, a, static_cast<S*>(status), p);
+void launch_rec_sweep(uint32_t tiles, unsigned cus, hipStream_t s, const SweepArgs<U>& a, void* status, int p) {
+  if (size_t(tiles) >= size_t(cus) * TS_SWEEP_PERSIST_TILES_PER_CU)
+    hipLaunchKernelGGL((k_sweep<U, VALS, S, false, false, true, RW>), dim3(cus),
+                       dim3(SweepShape<sizeof(U), false, VALS, false, RW>::threads), 0, s, a, static_cast<S*>(status), p);
+  else
+    hipLaunchKernelGGL((k_sweep<U, VALS, S, false, false, false, RW>), dim3(tiles),
+                       dim3(SweepShape<sizeof(U), false, VALS, false, RW>::threads), 0, s, a, static_cast<S*>(status), p);
 }
 
 template <class U, class S, bool LEARNED>
@@ -1370,13 +1376,13 @@ hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool g
     if (clk) clk->begin(s, gate ? "k_sweep_fallback" : rw ? "k_sweep_records" : "k_sweep", bytes);
     if (rw) {
       if (rw == 2 && vals)
-        launch_rec_sweep<U, S, true, 2>(tiles, s, a, j.status, p);
+        launch_rec_sweep<U, S, true, 2>(tiles, cus, s, a, j.status, p);
       else if (rw == 2)
-        launch_rec_sweep<U, S, false, 2>(tiles, s, a, j.status, p);
+        launch_rec_sweep<U, S, false, 2>(tiles, cus, s, a, j.status, p);
       else if (vals)
-        launch_rec_sweep<U, S, true, 4>(tiles, s, a, j.status, p);
+        launch_rec_sweep<U, S, true, 4>(tiles, cus, s, a, j.status, p);
       else
-        launch_rec_sweep<U, S, false, 4>(tiles, s, a, j.status, p);
+        launch_rec_sweep<U, S, false, 4>(tiles, cus, s, a, j.status, p);
     } else if constexpr (LEARNED) {
       hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U), true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
