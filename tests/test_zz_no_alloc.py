@@ -3,8 +3,10 @@
 Called through the C ABI and compared, bit for bit, with the existing
 ts_sort_with_buffer (and the oracle) on two calls in a row, the second with
 a smaller input, for 4- and 8-byte keys (the 8-byte case also through the
-divert and its finish); the context it runs on grows no workspace, and n
-beyond the stated maximum is refused from the host before any launch.
+divert and its finish), and at sizes whose scatter passes run as persistent
+grids against ts_sort_with_buffer on the device; the context it runs on
+grows no workspace, and n beyond the stated maximum is refused from the host
+before any launch.
 """
 from __future__ import annotations
 
@@ -49,6 +51,29 @@ def test_static_matches_sort_with_buffer_on_shrinking_calls(gpu, name, sizes):
             assert ctx.last_sort_info()["skipped_digits"] > 0  # diverted, finished through the static array
     ctx.check_device()
     assert ctx.workspace_bytes() == 0  # the context itself grew nothing
+
+
+@pytest.mark.parametrize("dtype,sizes", [(torch.int32, ((1 << 27) + 5, (1 << 24) + 43)),
+                                         (torch.uint64, ((1 << 26) + 15, 3_000_001))])
+def test_static_matches_sort_with_buffer_at_persistent_sizes(gpu, dtype, sizes):
+    """Large enough that the scatter passes run as persistent grids (the
+    first call; the second, smaller one launches a workgroup per tile), on
+    the device against ts_sort_with_buffer of the same keys."""
+    ctx = ts.Context(gpu)
+    ref = ts.Context(gpu)
+    for i, n in enumerate(sizes):
+        g = torch.Generator(device=gpu).manual_seed(40 + i)
+        lo, hi = (-(1 << 31), 1 << 31) if dtype == torch.int32 else (-(1 << 63), (1 << 63) - 1)
+        a = torch.randint(lo, hi, (n,), dtype=torch.int64 if dtype != torch.int32 else dtype, device=gpu,
+                          generator=g)
+        a = a.view(dtype) if dtype != torch.int32 else a
+        b = a.clone()
+        ctx.check(static_sort(ctx, a, torch.empty_like(a)))
+        ts.sort_with_buffer(b, torch.empty_like(b), ctx=ref)
+        torch.cuda.synchronize()
+        assert torch.equal(a.view(torch.uint8), b.view(torch.uint8)), (dtype, n)
+    ctx.check_device()
+    assert ctx.workspace_bytes() == 0
 
 
 def test_static_refuses_more_than_its_maximum(gpu):
