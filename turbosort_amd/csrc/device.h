@@ -51,6 +51,35 @@ template <class T> __device__ __forceinline__ T block256_exclusive_add(T v, T* s
   return off + inc - v;
 }
 
+// Inclusive prefix sum of a 32-bit value over the 64 lanes of a wave in
+// registers (DPP): shifts by 1, 2, 4 and 8 lanes inside each row of 16 (lanes
+// shifted in from outside the row read 0), then the last lane of each row
+// broadcast into the rows above it (row_bcast 15 into rows 1 and 3, then
+// row_bcast 31 into rows 2 and 3).  The value of wave_inclusive_add without
+// the LDS-crossbar shuffles.
+__device__ __forceinline__ uint32_t wave_inclusive_add_dpp(uint32_t v) {
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xf, 0xf, true));   // row_shr:1
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xf, 0xf, true));   // row_shr:2
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xf, 0xf, true));   // row_shr:4
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xf, 0xf, true));   // row_shr:8
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return v;
+}
+
+// block256_exclusive_add for 32-bit values with the wave scan in registers
+// and no closing barrier: the caller writes s_wave again only after a later
+// barrier.
+__device__ __forceinline__ uint32_t block256_exclusive_add_once(uint32_t v, uint32_t* s_wave) {
+  const unsigned tid = threadIdx.x, w = tid >> 6;
+  const uint32_t inc = wave_inclusive_add_dpp(v);
+  if ((tid & 63) == 63 && w < 4) s_wave[w] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+  for (unsigned i = 0; i < w && i < 4; ++i) off += s_wave[i];
+  return off + inc - v;
+}
+
 // The byte digit of a key at bit `shift` (a multiple of 8, uniform).
 template <class U> __device__ __forceinline__ uint32_t digit_of(U k, unsigned shift) {
   if constexpr (sizeof(U) == 8) {

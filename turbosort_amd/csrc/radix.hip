@@ -712,7 +712,8 @@ __device__ __forceinline__ void run_tile(
   }
   if (a.atomic_rank && dl)
     __hip_atomic_store(my, S(cnt) | (first_round ? kPre : kAgg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t excl_in_tile = block256_exclusive_add(cnt, L.wave);
+  // L.wave is written again only by the next tile, after several barriers.
+  const uint32_t excl_in_tile = block256_exclusive_add_once(cnt, L.wave);
   if (dl) {
 #pragma unroll
     for (int ww = 0; ww < NW; ++ww) L.whist[ww][tid] = excl_in_tile + wexcl[ww];
