@@ -37,20 +37,6 @@ template <class U> __device__ __forceinline__ void opaque(U& x) {
   }
 }
 
-// Exclusive scan of one value per lane over the first 256 lanes of the
-// workgroup (every lane of the workgroup must call it; s_wave holds four
-// slots, one per wave of those lanes).
-template <class T> __device__ __forceinline__ T block256_exclusive_add(T v, T* s_wave) {
-  const unsigned tid = threadIdx.x, w = tid >> 6;
-  T inc = wave_inclusive_add(v);
-  if ((tid & 63) == 63 && w < 4) s_wave[w] = inc;
-  __syncthreads();
-  T off = 0;
-  for (unsigned i = 0; i < w && i < 4; ++i) off += s_wave[i];
-  __syncthreads();
-  return off + inc - v;
-}
-
 // Inclusive prefix sum of a 32-bit value over the 64 lanes of a wave in
 // registers (DPP): shifts by 1, 2, 4 and 8 lanes inside each row of 16 (lanes
 // shifted in from outside the row read 0), then the last lane of each row
@@ -65,6 +51,24 @@ __device__ __forceinline__ uint32_t wave_inclusive_add_dpp(uint32_t v) {
   v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xa, 0xf, false));  // row_bcast:15
   v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xc, 0xf, false));  // row_bcast:31
   return v;
+}
+
+// Exclusive scan of one value per lane over the first 256 lanes of the
+// workgroup (every lane of the workgroup must call it; s_wave holds four
+// slots, one per wave of those lanes).
+template <class T> __device__ __forceinline__ T block256_exclusive_add(T v, T* s_wave) {
+  const unsigned tid = threadIdx.x, w = tid >> 6;
+  T inc;
+  if constexpr (std::is_same<T, uint32_t>::value)
+    inc = wave_inclusive_add_dpp(v);
+  else
+    inc = wave_inclusive_add(v);
+  if ((tid & 63) == 63 && w < 4) s_wave[w] = inc;
+  __syncthreads();
+  T off = 0;
+  for (unsigned i = 0; i < w && i < 4; ++i) off += s_wave[i];
+  __syncthreads();
+  return off + inc - v;
 }
 
 // block256_exclusive_add for 32-bit values with the wave scan in registers

@@ -352,7 +352,7 @@ template <class U, int T, int IPT, bool VALS> __global__ __launch_bounds__(T) vo
         r[x] = L.cnt[rb + x];
         sum += r[x];
       }
-      const uint32_t inc = wave_inclusive_add(sum);
+      const uint32_t inc = wave_inclusive_add_dpp(sum);
       if (lane == 63) L.wsum[w] = inc;
       __syncthreads();
       uint32_t wpre = 0, all = 0;
@@ -459,7 +459,7 @@ template <class U, int T, int IPT, bool VALS> struct SegByteLds {
 // Exclusive scan of one value per lane over all T lanes of the workgroup.
 template <int T> __device__ __forceinline__ uint32_t block_exclusive_add(uint32_t v, uint32_t* s_wave) {
   const unsigned tid = threadIdx.x, w = tid >> 6;
-  const uint32_t inc = wave_inclusive_add(v);
+  const uint32_t inc = wave_inclusive_add_dpp(v);
   if ((tid & 63) == 63) s_wave[w] = inc;
   __syncthreads();
   uint32_t off = 0;
