@@ -591,15 +591,16 @@ constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
 // Everything a tile does after it knows where it is: load, publish its digit
 // counts, rank, look back, stage, store.  FULL tiles (all but each segment's
 // last) run without any per-key bounds test.  LOOP: the workgroup goes on to
-// another tile afterwards (a persistent grid); it then claims the next round
-// of segment sg0 (nt0 rounds) while its stores go out, and leaves the claim,
-// or kNoClaim, in *next.
+// another tile afterwards (a persistent grid); it then clears the counters
+// for that tile and claims the next round of segment sg0 (nt0 rounds) while
+// its stores go out, and leaves the claim, or kNoClaim, in L.ticket for the
+// barrier that ends the tile.
 template <class U, bool VALS, class S, bool FULL, bool LEARNED, bool SMALL, int RW, bool LOOP = false>
 __device__ __forceinline__ void run_tile(
     SweepLds<U, VALS, SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt, LEARNED, SMALL, RW>& L,
     const SweepArgs<U>& a, const LearnedCoord& mc, S* __restrict__ status, int pass, unsigned shift, bool first,
     bool last, size_t tile_base, uint32_t valid, size_t first_slot, size_t slot, bool first_round,
-    unsigned long long digit_base, uint32_t sg0 = 0, uint32_t nt0 = 0, uint32_t* next = nullptr) {
+    unsigned long long digit_base, uint32_t sg0 = 0, uint32_t nt0 = 0) {
   using R = typename RecOf<RW>::T;
   constexpr int IPT = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::ipt;
   constexpr int THREADS = SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads;
@@ -784,6 +785,12 @@ __device__ __forceinline__ void run_tile(
   }
   if (dl) L.gofs[tid] = digit_base + before_tiles - excl_in_tile;
   __syncthreads();
+  if constexpr (LOOP) {
+    // The next tile's counters: this tile reads them no more (staging is
+    // done), so they are cleared while its stores go out.
+    for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
+    if (tid < 256) L.tcnt[tid] = 0;
+  }
 
   // Write each digit's run of the tile contiguously.  The output transform
   // and whether keys are written are uniform, so each case is its own loop.
@@ -830,7 +837,7 @@ __device__ __forceinline__ void run_tile(
       }
       if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
-    if (kEarlyClaim && tid == 0) *next = (nt0 != 0 && claimed < nt0) ? ((sg0 << 24) | claimed) : kNoClaim;
+    if (kEarlyClaim && tid == 0) L.ticket = (nt0 != 0 && claimed < nt0) ? ((sg0 << 24) | claimed) : kNoClaim;
     (void)TILE;
     return;
   }
@@ -844,7 +851,7 @@ __device__ __forceinline__ void run_tile(
     store([](U x) { return from_key(x, kFloat); }, true);
   else
     store([](U x) { return x; }, true);
-  if (kEarlyClaim && tid == 0) *next = (nt0 != 0 && claimed < nt0) ? ((sg0 << 24) | claimed) : kNoClaim;
+  if (kEarlyClaim && tid == 0) L.ticket = (nt0 != 0 && claimed < nt0) ? ((sg0 << 24) | claimed) : kNoClaim;
   (void)TILE;
 }
 
@@ -917,36 +924,36 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
       next_row[size_t(blockIdx.x) * 256 + tid] = 0;
     }
   }
-  uint32_t next = kNoClaim;  // lane 0: the claim made during the last tile
+  for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
+  if (tid < 256) L.tcnt[tid] = 0;
+  bool have = false;  // (uniform) L.ticket holds the claim made during the last tile
   bool first_claim = true;
   while (true) {
-    for (unsigned i = tid; i < NW * 256; i += THREADS) (&L.whist[0][0])[i] = 0;
-    if (tid < 256) L.tcnt[tid] = 0;
-    if (tid == 0 && next != kNoClaim) {
-      L.ticket = next;
-    } else if (tid == 0) {
-      uint32_t got = kSegments, r = 0;
-      // Past a workgroup's first claim its own segment is exhausted: read
-      // every counter at once rather than try each one in turn.
-      uint32_t seen[kSegments];
+    if (!have) {
+      if (tid == 0) {
+        uint32_t got = kSegments, r = 0;
+        // Past a workgroup's first claim its own segment is exhausted: read
+        // every counter at once rather than try each one in turn.
+        uint32_t seen[kSegments];
 #pragma unroll
-      for (int q = 0; q < kSegments; ++q)
-        seen[q] = first_claim ? 0u
-                              : __hip_atomic_load(&a.seg_tickets[pass * kSegments + q], __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-      for (uint32_t i = 0; i < uint32_t(kSegments); ++i) {
-        const uint32_t sg = (xcc + i) & uint32_t(kSegments - 1);
-        if (nt[sg] == 0 || seen[sg] >= nt[sg]) continue;
-        r = atomicAdd(&a.seg_tickets[pass * kSegments + sg], 1u);
-        if (r < nt[sg]) {
-          got = sg;
-          break;
+        for (int q = 0; q < kSegments; ++q)
+          seen[q] = first_claim ? 0u
+                                : __hip_atomic_load(&a.seg_tickets[pass * kSegments + q], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        for (uint32_t i = 0; i < uint32_t(kSegments); ++i) {
+          const uint32_t sg = (xcc + i) & uint32_t(kSegments - 1);
+          if (nt[sg] == 0 || seen[sg] >= nt[sg]) continue;
+          r = atomicAdd(&a.seg_tickets[pass * kSegments + sg], 1u);
+          if (r < nt[sg]) {
+            got = sg;
+            break;
+          }
         }
+        L.ticket = (got << 24) | r;
       }
-      L.ticket = (got << 24) | r;
+      first_claim = false;
+      __syncthreads();
     }
-    first_claim = false;
-    __syncthreads();
     const uint32_t claim = __builtin_amdgcn_readfirstlane(L.ticket);
     const uint32_t seg = claim >> 24;
     if (seg >= uint32_t(kSegments)) return;
@@ -968,13 +975,16 @@ __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::t
     if (valid == uint32_t(TILE))
       run_tile<U, VALS, S, true, LEARNED, SMALL, RW, PERSIST>(L, a, mc, status, pass, shift, first, last, tile_base,
                                                                 valid, first_slot, slot, round == 0, digit_base, seg,
-                                                                seg_tiles, &next);
+                                                                seg_tiles);
     else
       run_tile<U, VALS, S, false, LEARNED, SMALL, RW, PERSIST>(L, a, mc, status, pass, shift, first, last, tile_base,
                                                                  valid, first_slot, slot, round == 0, digit_base, seg,
-                                                                 seg_tiles, &next);
+                                                                 seg_tiles);
     if (!PERSIST) return;
-    __syncthreads();  // the tile's LDS is reused by the next claim
+    // The tile's LDS is reused by the next tile, whose claim lane 0 has left
+    // in L.ticket.
+    __syncthreads();
+    have = __builtin_amdgcn_readfirstlane(L.ticket) != kNoClaim;
   }
 }
 
