@@ -53,6 +53,52 @@ __device__ __forceinline__ uint32_t wave_inclusive_add_dpp(uint32_t v) {
   return v;
 }
 
+// One step of a DPP wave scan: each lane combines its value with the one
+// that CTRL moves to it; lanes whose source is outside its row, and rows
+// outside the row mask RM, take `identity`.  64-bit values move as two
+// halves under the same control.
+template <int CTRL, int RM, class U, class Op>
+__device__ __forceinline__ U dpp_combine(U v, U identity, Op op) {
+  if constexpr (sizeof(U) == 8) {
+    const uint32_t lo = uint32_t(
+        __builtin_amdgcn_update_dpp(int(uint32_t(identity)), int(uint32_t(v)), CTRL, RM, 0xf, false));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_update_dpp(int(uint32_t(uint64_t(identity) >> 32)),
+                                                             int(uint32_t(uint64_t(v) >> 32)), CTRL, RM, 0xf, false));
+    return op(v, U((uint64_t(hi) << 32) | lo));
+  } else {
+    return op(v, U(uint32_t(
+                     __builtin_amdgcn_update_dpp(int(uint32_t(identity)), int(uint32_t(v)), CTRL, RM, 0xf, false))));
+  }
+}
+
+// An associative op with `identity` over the 64 lanes of a wave by the DPP
+// scan of wave_inclusive_add_dpp: lane 63 ends with the whole wave's value,
+// which every lane gets.  Every lane of the wave must be active.
+template <class U, class Op> __device__ __forceinline__ U wave_reduce_dpp(U v, U identity, Op op) {
+  v = dpp_combine<0x111, 0xf>(v, identity, op);  // row_shr:1
+  v = dpp_combine<0x112, 0xf>(v, identity, op);  // row_shr:2
+  v = dpp_combine<0x114, 0xf>(v, identity, op);  // row_shr:4
+  v = dpp_combine<0x118, 0xf>(v, identity, op);  // row_shr:8
+  v = dpp_combine<0x142, 0xa>(v, identity, op);  // row_bcast:15
+  v = dpp_combine<0x143, 0xc>(v, identity, op);  // row_bcast:31
+  if constexpr (sizeof(U) == 8) {
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), 63));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(uint64_t(v) >> 32)), 63));
+    return U((uint64_t(hi) << 32) | lo);
+  } else {
+    return U(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), 63)));
+  }
+}
+template <class U> __device__ __forceinline__ U wave_max_dpp(U v) {
+  return wave_reduce_dpp(v, U(0), [](U a, U b) { return a > b ? a : b; });
+}
+template <class U> __device__ __forceinline__ U wave_min_dpp(U v) {
+  return wave_reduce_dpp(v, U(~U(0)), [](U a, U b) { return a < b ? a : b; });
+}
+template <class U> __device__ __forceinline__ U wave_or_dpp(U v) {
+  return wave_reduce_dpp(v, U(0), [](U a, U b) { return U(a | b); });
+}
+
 // Exclusive scan of one value per lane over the first 256 lanes of the
 // workgroup (every lane of the workgroup must call it; s_wave holds four
 // slots, one per wave of those lanes).

@@ -57,11 +57,7 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
-template <class U> __device__ __forceinline__ U wave_or(U v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) v |= __shfl_xor(v, d, 64);
-  return v;
-}
+template <class U> __device__ __forceinline__ U wave_or(U v) { return wave_or_dpp(v); }
 
 // Zeroes the class counters (a kernel rather than a memset so that the whole
 // sort can be captured in a graph).
@@ -709,13 +705,7 @@ __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
     __syncthreads();
     {
       const uint64_t x = wave_or<uint64_t>(uint64_t(vary));
-      uint64_t a0 = uint64_t(mn), a1 = uint64_t(mx_k);
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t o0 = __shfl_xor(a0, d, 64), o1 = __shfl_xor(a1, d, 64);
-        a0 = o0 < a0 ? o0 : a0;
-        a1 = o1 > a1 ? o1 : a1;
-      }
+      const uint64_t a0 = wave_min_dpp(uint64_t(mn)), a1 = wave_max_dpp(uint64_t(mx_k));
       if (lane == 0 && x) {
         atomicOr(&L.flags[0], uint32_t(x));
         if (P == 8) atomicOr(&L.flags[1], uint32_t(x >> 32));
@@ -753,8 +743,7 @@ __global__ __launch_bounds__(T) void k_seg_msd(SegArgs a, int cls) {
       sum += c[q];
       mx = max(mx, c[q]);
     }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) mx = max(mx, uint32_t(__shfl_xor(mx, d, 64)));
+    mx = wave_max_dpp(mx);
     if (lane == 0) atomicMax(&L.flags[2], mx);
     uint32_t run = block_exclusive_add<T>(sum, L.wsum);
 #pragma unroll
@@ -1003,11 +992,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_s
           mn = min(mn, k[j]);
           mx = max(mx, k[j]);
         }
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        mn = min(mn, U(__shfl_xor(mn, d, 64)));
-        mx = max(mx, U(__shfl_xor(mx, d, 64)));
-      }
+      mn = wave_min_dpp(mn);
+      mx = wave_max_dpp(mx);
       if (lane == 0) {
         atomicMin(&L.kmin, mn);
         atomicMax(&L.kmax, mx);
@@ -1047,8 +1033,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_s
           sum += (wv & 0xFFFFu) + (wv >> 16);
           big = max(big, max(wv & 0xFFFFu, wv >> 16));
         }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) big = max(big, uint32_t(__shfl_xor(big, d, 64)));
+        big = wave_max_dpp(big);
         if (lane == 0) atomicMax(&L.flags[1], big);
         uint32_t run = block_exclusive_add<T>(sum, L.wsum);
 #pragma unroll
@@ -1127,8 +1112,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_s
             if (p < len) vary |= k[j] ^ kmin;
             L.kv[p] = p < len ? make_uint2(k[j], v[j]) : make_uint2(~0u, 0u);
           }
-#pragma unroll
-          for (int d = 1; d < 64; d <<= 1) vary |= U(__shfl_xor(vary, d, 64));
+          vary = wave_or_dpp(vary);
           if (lane == 0) atomicOr(&L.flags[0], vary);
         }
         __syncthreads();
@@ -1237,12 +1221,8 @@ __global__ __launch_bounds__(T) void k_seg_msd_pipe(SegArgs a, int cls) {
           mn = k[j] < mn ? k[j] : mn;
           mx = k[j] > mx ? k[j] : mx;
         }
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const U o0 = U(__shfl_xor(mn, d, 64)), o1 = U(__shfl_xor(mx, d, 64));
-        mn = o0 < mn ? o0 : mn;
-        mx = o1 > mx ? o1 : mx;
-      }
+      mn = wave_min_dpp(mn);
+      mx = wave_max_dpp(mx);
       if (lane == 0) {
         atomicMin(&L.kmin, mn);
         atomicMax(&L.kmax, mx);
@@ -1281,8 +1261,7 @@ __global__ __launch_bounds__(T) void k_seg_msd_pipe(SegArgs a, int cls) {
           sum += (wv & 0xFFFFu) + (wv >> 16);
           big = max(big, max(wv & 0xFFFFu, wv >> 16));
         }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) big = max(big, uint32_t(__shfl_xor(big, d, 64)));
+        big = wave_max_dpp(big);
         if (lane == 0) atomicMax(&L.flags[2], big);
         uint32_t run = block_exclusive_add<T>(sum, L.wsum);
 #pragma unroll
