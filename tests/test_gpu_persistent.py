@@ -1,11 +1,12 @@
 """Sorts large enough for the persistent scatter passes, key for key.
 
-Keys-only sorts of 4- and 8-byte keys, and record-carrying by-key sorts, run
-their scatter passes as persistent grids (one workgroup per CU claiming
-tiles, each claiming its next tile on the segment it works on while its
-stores go out) from 12 tiles per CU on: about 1.0e8 4-byte keys, 5.7e7
-8-byte keys or 2.2e7 16-byte records with their positions on 256 CUs.  These sizes are
-too large for the oracle to sort in seconds, so each output is compared with
+Keys-only sorts of 4- and 8-byte keys, record-carrying by-key sorts and the
+learned sort's partition run their scatter passes as persistent grids (one
+workgroup per CU claiming tiles, each claiming its next tile on the segment
+it works on while its stores go out) from 12 tiles per CU on: about 1.0e8
+4-byte keys, 5.7e7 8-byte keys, 2.2e7 16-byte records with their positions
+or 5.0e7 learned keys on 256 CUs.  These sizes are too large for the oracle
+to sort in seconds, so each output is compared with
 torch.sort of the same keys on the device (a keys-only sort has one correct
 output), after a check that the path under test ran (executed passes).
 Cases: uniform u32 at the headline size, i32 and f32 with ragged last tiles
@@ -13,8 +14,9 @@ Cases: uniform u32 at the headline size, i32 and f32 with ragged last tiles
 bits of every byte are zero (all keys in one value segment from the second
 pass on, so every workgroup but those of one XCD finds its own segment empty
 and takes tiles elsewhere), skewed u64 (a lognormal mixture: unequal
-segments, workgroups moving between them), and 16-byte records carried with
-their positions (the by-key passes), checked against a stable torch.sort.
+segments, workgroups moving between them), 16-byte records carried with
+their positions (the by-key passes), checked against a stable torch.sort,
+and the learned sort's bucket-digit passes on the lognormal mixture.
 """
 from __future__ import annotations
 
@@ -137,3 +139,23 @@ def test_carried_records_with_order(gpu):
     torch.cuda.synchronize()
     assert torch.equal(order, want)
     assert torch.equal(rec, src[want])
+
+
+@pytest.mark.parametrize("as_f64", [False, True])
+def test_learned_partition_at_persistent_size(gpu, as_f64):
+    """The learned sort's bucket-digit passes run as persistent grids from 12
+    tiles of 16384 keys per CU (about 5e7 keys): 6e7 keys of the lognormal
+    mixture with hot values, at the default 1 % sample, judged; the output
+    must equal torch.sort of the same keys."""
+    import bench
+
+    c = ts.Context(gpu)
+    n = 60_000_000
+    assert n >= persistent_size(gpu, 1024 * 16)
+    t = bench.lognormal_keys_dev(n, 23, c, as_f64=as_f64)
+    want = torch.sort(t).values if as_f64 else torch.sort(as_signed_order(t)).values
+    rep = ts.learned_sort(t, config=ts.LearnedConfig(sample_per_mille=10), ctx=c)
+    torch.cuda.synchronize()
+    assert rep["verified"] == 1, rep
+    assert torch.equal(t if as_f64 else as_signed_order(t), want)
+    c.check_device()

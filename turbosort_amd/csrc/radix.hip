@@ -877,11 +877,12 @@ __device__ __forceinline__ void run_tile(
 // so that the records end in order without a gather.  PERSIST: a grid
 // smaller than the tile count whose workgroups claim tiles until none is left:
 // the gated second sort, which then costs a few microseconds when it does not
-// run, and the keys-only and record-carrying passes of large sorts, where a
-// workgroup claims its next tile while its stores go out instead of ending
-// and a new one starting (MI355X, 2^28 u32 keys: 2.36 -> 2.27 ms per sort;
-// 10^8 16-byte records by key 3.56 -> 3.53 ms; below about 12 tiles per CU
-// the end of the pass costs more than that saves).
+// run, and the keys-only, record-carrying and learned passes of large
+// sorts, where a workgroup claims its next tile while its stores go out
+// instead of ending and a new one starting (MI355X, 2^28 u32 keys: 2.36 ->
+// 2.27 ms per sort; 10^8 16-byte records by key 3.56 -> 3.53 ms; 10^9
+// learned keys about 3 %; below about 12 tiles per CU the end of the pass
+// costs more than that saves).
 template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false, bool PERSIST = false, int RW = 0>
 __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads)) void k_sweep(
     SweepArgs<U> a, S* __restrict__ status, int pass) {
@@ -1395,8 +1396,12 @@ hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool g
       else
         launch_rec_sweep<U, S, false, 4>(tiles, cus, s, a, j.status, p);
     } else if constexpr (LEARNED) {
-      hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U), true>::threads), 0, s, a,
-                         static_cast<S*>(j.status), p);
+      if (size_t(tiles) >= size_t(cus) * TS_SWEEP_PERSIST_TILES_PER_CU)
+        hipLaunchKernelGGL((k_sweep<U, false, S, true, false, true>), dim3(cus),
+                           dim3(SweepShape<sizeof(U), true>::threads), 0, s, a, static_cast<S*>(j.status), p);
+      else
+        hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U), true>::threads), 0, s,
+                           a, static_cast<S*>(j.status), p);
     } else if (vals) {
       hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U), false, true>::threads), 0, s, a,
                          static_cast<S*>(j.status), p);
