@@ -14,8 +14,9 @@ Cases: uniform u32 at the headline size, i32 and f32 with ragged last tiles
 bits of every byte are zero (all keys in one value segment from the second
 pass on, so every workgroup but those of one XCD finds its own segment empty
 and takes tiles elsewhere), skewed u64 (a lognormal mixture: unequal
-segments, workgroups moving between them), 16-byte records carried with
-their positions (the by-key passes), checked against a stable torch.sort,
+segments, workgroups moving between them), 16- and 8-byte records carried
+with their positions (the by-key passes), checked against a stable
+torch.sort,
 and the learned sort's bucket-digit passes on the lognormal mixture.
 """
 from __future__ import annotations
@@ -110,13 +111,15 @@ def test_skewed_u64(gpu):
     c.check_device()
 
 
-def test_carried_records_with_order(gpu):
+@pytest.mark.parametrize("rec_bytes,tile", [(16, 512 * 14), (8, 512 * 24)])
+def test_carried_records_with_order(gpu, rec_bytes, tile):
     """The by-key shape at persistent size: 16-byte records {u32 id, f32
-    score, u64 payload} keyed on the score (duplicates, NaN, -0.0, inf),
-    carried through the passes with their positions.  order() must be the
-    stable argsort of the scores' radix keys (torch.sort with stable=True of
-    the same keys), and the records must end in that order."""
-    n = persistent_size(gpu, 512 * 14) + 1001
+    score, u64 payload}, or 8-byte records {u32 id, f32 score}, keyed on the
+    score (duplicates, NaN, -0.0, inf), carried through the passes with their
+    positions.  order() must be the stable argsort of the scores' radix keys
+    (torch.sort with stable=True of the same keys), and the records must end
+    in that order."""
+    n = persistent_size(gpu, tile) + 1001
     g = torch.Generator(device=gpu).manual_seed(14)
     ids = torch.arange(n, dtype=torch.int32, device=gpu)
     score = torch.randn(n, generator=g, device=gpu)
@@ -124,11 +127,12 @@ def test_carried_records_with_order(gpu):
     score[::997] = float("nan")
     score[1::1009] = -0.0
     score[2::1013] = float("inf")
-    payload = torch.randint(-(1 << 62), 1 << 62, (n,), generator=g, device=gpu)
-    rec = torch.empty(n, 16, dtype=torch.uint8, device=gpu)
+    rec = torch.empty(n, rec_bytes, dtype=torch.uint8, device=gpu)
     rec[:, 0:4] = ids.view(torch.uint8).reshape(n, 4)
     rec[:, 4:8] = score.view(torch.uint8).reshape(n, 4)
-    rec[:, 8:16] = payload.view(torch.uint8).reshape(n, 8)
+    if rec_bytes == 16:
+        payload = torch.randint(-(1 << 62), 1 << 62, (n,), generator=g, device=gpu)
+        rec[:, 8:16] = payload.view(torch.uint8).reshape(n, 8)
     src = rec.clone()
     buf = ts.SortByKeyBuffer()
     ts.sort_by_key_with_buffer(rec, buf, ts.KeyField(4, torch.float32))
