@@ -3,9 +3,9 @@
 Keys-only sorts of 4- and 8-byte keys, record-carrying by-key sorts and the
 learned sort's partition run their scatter passes as persistent grids (one
 workgroup per CU claiming tiles, each claiming its next tile on the segment
-it works on while its stores go out) from 12 tiles per CU on: about 1.0e8
-4-byte keys, 5.7e7 8-byte keys, 2.2e7 16-byte records with their positions
-or 5.0e7 learned keys on 256 CUs.  These sizes are too large for the oracle
+it works on while its stores go out) from 8 tiles per CU on: about 6.7e7
+4-byte keys, 3.8e7 8-byte keys, 1.5e7 16-byte records with their positions
+or 3.4e7 learned keys on 256 CUs.  These sizes are too large for the oracle
 to sort in seconds, so each output is compared with
 torch.sort of the same keys on the device (a keys-only sort has one correct
 output), after a check that the path under test ran (executed passes).
@@ -28,7 +28,7 @@ import turbosort_amd as ts
 
 pytestmark = pytest.mark.gpu
 
-PERSIST_TILES_PER_CU = 12
+PERSIST_TILES_PER_CU = 8
 
 
 def persistent_size(gpu, tile: int) -> int:
@@ -147,8 +147,8 @@ def test_carried_records_with_order(gpu, rec_bytes, tile):
 
 @pytest.mark.parametrize("as_f64", [False, True])
 def test_learned_partition_at_persistent_size(gpu, as_f64):
-    """The learned sort's bucket-digit passes run as persistent grids from 12
-    tiles of 16384 keys per CU (about 5e7 keys): 6e7 keys of the lognormal
+    """The learned sort's bucket-digit passes run as persistent grids from 8
+    tiles of 16384 keys per CU (about 3.4e7 keys): 6e7 keys of the lognormal
     mixture with hot values, at the default 1 % sample, judged; the output
     must equal torch.sort of the same keys."""
     import bench

@@ -881,7 +881,7 @@ __device__ __forceinline__ void run_tile(
 // sorts, where a workgroup claims its next tile while its stores go out
 // instead of ending and a new one starting (MI355X, 2^28 u32 keys: 2.36 ->
 // 2.27 ms per sort; 10^8 16-byte records by key 3.56 -> 3.53 ms; 10^9
-// learned keys about 3 %; below about 12 tiles per CU the end of the pass
+// learned keys about 3 %; below about 8 tiles per CU the end of the pass
 // costs more than that saves).
 template <class U, bool VALS, class S, bool LEARNED = false, bool SMALL = false, bool PERSIST = false, int RW = 0>
 __global__ __launch_bounds__((SweepShape<sizeof(U), LEARNED, VALS, SMALL, RW>::threads)) void k_sweep(
@@ -1334,12 +1334,13 @@ size_t first_segment(size_t n, int key_bytes, bool learned, bool vals, int rec_w
 #ifndef TS_SWEEP_PERSIST_FROM
 #define TS_SWEEP_PERSIST_FROM 4
 #endif
-// Keys-only passes of 4- and 8-byte keys (one workgroup per CU) run as
-// persistent grids from this many tiles per CU on (MI355X: u32 2^26 keys, 8
-// tiles per CU, 0.715 -> 0.704 ms; 2^27 1.266 -> 1.213 ms; u64 2^25, 8 tiles
-// per CU, 0.854 -> 0.861 ms, 2^27 2.700 -> 2.648 ms).
+// Keys-only, record-carrying and learned passes (one workgroup per CU) run
+// as persistent grids from this many tiles per CU on (MI355X: u32 2^26 keys,
+// 8 tiles per CU, 0.711 -> 0.686 ms, 2^27 1.266 -> 1.213 ms; 2^25, 4 tiles
+// per CU, 0.415 -> 0.418 ms with a threshold of 4; u64 2^27 2.700 -> 2.648
+// ms).
 #ifndef TS_SWEEP_PERSIST_TILES_PER_CU
-#define TS_SWEEP_PERSIST_TILES_PER_CU 12
+#define TS_SWEEP_PERSIST_TILES_PER_CU 8
 #endif
 
 // gate: the second sort after a divert (persistent launches for keys-only
