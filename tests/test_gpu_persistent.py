@@ -1,12 +1,13 @@
 """Sorts large enough for the persistent scatter passes, key for key.
 
-Keys-only sorts of 4- and 8-byte keys, record-carrying by-key sorts and the
-learned sort's partition run their scatter passes as persistent grids (one
-workgroup per CU claiming tiles, each claiming its next tile on the segment
-it works on while its stores go out) from 8 tiles per CU on: about 6.7e7
-4-byte keys, 3.8e7 8-byte keys, 1.5e7 16-byte records with their positions
-or 3.4e7 learned keys on 256 CUs.  These sizes are too large for the oracle
-to sort in seconds, so each output is compared with
+Keys-only and key-value sorts of 4- and 8-byte keys, record-carrying by-key
+sorts and the learned sort's partition run their scatter passes as
+persistent grids (one workgroup per CU claiming tiles, each claiming its
+next tile on the segment it works on while its stores go out) from 8 tiles
+per CU on: about 6.7e7 4-byte keys, 3.8e7 8-byte keys, 1.5e7 16-byte
+records with their positions or 3.4e7 learned keys on 256 CUs.  These sizes
+are too large for the oracle to sort in seconds, so each output is compared
+with
 torch.sort of the same keys on the device (a keys-only sort has one correct
 output), after a check that the path under test ran (executed passes).
 Cases: uniform u32 at the headline size, i32 and f32 with ragged last tiles
@@ -17,7 +18,8 @@ and takes tiles elsewhere), skewed u64 (a lognormal mixture: unequal
 segments, workgroups moving between them), 16- and 8-byte records carried
 with their positions (the by-key passes), checked against a stable
 torch.sort,
-and the learned sort's bucket-digit passes on the lognormal mixture.
+the learned sort's bucket-digit passes on the lognormal mixture, and the
+(key, position) passes of a by-key sort keyed by a separate array.
 """
 from __future__ import annotations
 
@@ -163,3 +165,23 @@ def test_learned_partition_at_persistent_size(gpu, as_f64):
     assert rep["verified"] == 1, rep
     assert torch.equal(t if as_f64 else as_signed_order(t), want)
     c.check_device()
+
+
+@pytest.mark.parametrize("key_dtype,tile", [(torch.int32, 512 * 36), (torch.float64, 512 * 24)])
+def test_keys_of_their_own_with_positions(gpu, key_dtype, tile):
+    """A by-key sort keyed by a separate array sorts (key, position) pairs:
+    the key-value passes at persistent size.  The records (their indices)
+    must end in the stable order of the keys (torch.sort with stable=True;
+    duplicated keys, and no NaN or zero, whose orders differ in torch)."""
+    n = persistent_size(gpu, tile) + 4321
+    g = torch.Generator(device=gpu).manual_seed(15)
+    if key_dtype == torch.int32:
+        keys = torch.randint(-(1 << 20), 1 << 20, (n,), dtype=torch.int32, device=gpu, generator=g)
+    else:
+        keys = torch.randn(n, dtype=torch.float64, device=gpu, generator=g).round(decimals=3)
+        keys[keys == 0] = 0.5
+    ids = torch.arange(n, dtype=torch.int32, device=gpu)
+    want = torch.sort(keys, stable=True).indices.to(torch.int32)
+    ts.sort_by_key(ids, lambda _: keys)
+    torch.cuda.synchronize()
+    assert torch.equal(ids, want)

@@ -1334,11 +1334,11 @@ size_t first_segment(size_t n, int key_bytes, bool learned, bool vals, int rec_w
 #ifndef TS_SWEEP_PERSIST_FROM
 #define TS_SWEEP_PERSIST_FROM 4
 #endif
-// Keys-only, record-carrying and learned passes (one workgroup per CU) run
-// as persistent grids from this many tiles per CU on (MI355X: u32 2^26 keys,
-// 8 tiles per CU, 0.711 -> 0.686 ms, 2^27 1.266 -> 1.213 ms; 2^25, 4 tiles
-// per CU, 0.415 -> 0.418 ms with a threshold of 4; u64 2^27 2.700 -> 2.648
-// ms).
+// Keys-only, key-value, record-carrying and learned passes (one workgroup
+// per CU) run as persistent grids from this many tiles per CU on (MI355X:
+// u32 2^26 keys, 8 tiles per CU, 0.711 -> 0.686 ms, 2^27 1.266 -> 1.213 ms;
+// 2^25, 4 tiles per CU, 0.415 -> 0.418 ms with a threshold of 4; u64 2^27
+// 2.700 -> 2.648 ms).
 #ifndef TS_SWEEP_PERSIST_TILES_PER_CU
 #define TS_SWEEP_PERSIST_TILES_PER_CU 8
 #endif
@@ -1404,8 +1404,14 @@ hipError_t run_sweeps(const RadixJob& j, hipStream_t s, KernelClock* clk, bool g
         hipLaunchKernelGGL((k_sweep<U, false, S, true>), dim3(tiles), dim3(SweepShape<sizeof(U), true>::threads), 0, s,
                            a, static_cast<S*>(j.status), p);
     } else if (vals) {
-      hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U), false, true>::threads), 0, s, a,
-                         static_cast<S*>(j.status), p);
+      if (sizeof(U) >= 4 && size_t(tiles) >= size_t(cus) * TS_SWEEP_PERSIST_TILES_PER_CU) {
+        if constexpr (sizeof(U) >= 4)
+          hipLaunchKernelGGL((k_sweep<U, true, S, false, false, true>), dim3(cus),
+                             dim3(SweepShape<sizeof(U), false, true>::threads), 0, s, a, static_cast<S*>(j.status), p);
+      } else {
+        hipLaunchKernelGGL((k_sweep<U, true, S>), dim3(tiles), dim3(SweepShape<sizeof(U), false, true>::threads), 0, s,
+                           a, static_cast<S*>(j.status), p);
+      }
     } else if (gate || (divert && p >= TS_SWEEP_PERSIST_FROM)) {
       if constexpr (sizeof(U) == 8) {
         const unsigned grid = unsigned(std::min<size_t>(tiles, size_t(j.num_cus)));
